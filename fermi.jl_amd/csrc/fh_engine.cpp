@@ -1,0 +1,1034 @@
+// Host side of libfermihip: device memory, the tensor-contraction engine, the closed-shell
+// DF-CCSD residual, the DIIS-accelerated iteration and the (T) driver, plus the C ABI.
+//
+// The CCSD residual is a factorised form (Fov/Foo/Fvv, Loo/Lvv, Woooo, Wvoov/Wvovo
+// intermediates) of the closed-shell term list of Fermi.jl's RCCSDHelper.jl:57-220; it is
+// the same function of (T1, T2) and is checked against a term-by-term restatement of that
+// file.  The T2 numerator is accumulated as P and symmetrised at the end,
+// newT2 = P + P[j,i,b,a] (RCCSDHelper.jl:190).
+#include "fh_internal.h"
+#include "../../include/fermihip.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <sstream>
+
+namespace fh {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& m) { g_last_error = m; }
+
+// ------------------------------------------------------------------ pool
+Pool::~Pool() {
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(device_);
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  for (auto& kv : live_) (void)hipFree(kv.first);
+  (void)hipSetDevice(cur);
+}
+void* Pool::get(size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  auto it = free_.lower_bound(bytes);
+  // reuse a free block of at most 25% more than asked
+  if (it != free_.end() && it->first <= bytes + bytes / 4) {
+    void* p = it->second;
+    size_t sz = it->first;
+    free_.erase(it);
+    live_[p] = sz;
+    in_use_ += sz;
+    peak_ = std::max(peak_, in_use_);
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    release_free();
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error{-4, "out of device memory allocating " + std::to_string(bytes) + " bytes"};
+    }
+  }
+  live_[p] = bytes;
+  in_use_ += bytes;
+  peak_ = std::max(peak_, in_use_);
+  return p;
+}
+void Pool::put(void* p) {
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  in_use_ -= it->second;
+  free_.insert({it->second, p});
+  live_.erase(it);
+}
+void Pool::release_free() {
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+}
+
+// ------------------------------------------------------------------ context
+struct Phase {
+  double ms = 0.0;
+  double flops = 0.0;
+  int calls = 0;
+};
+
+struct IterRow {
+  int ite;
+  double e, de, rms, ms;
+  bool diis;
+};
+
+// A timed span recorded on the context stream (hipEvent pair), read back at the next
+// synchronisation point by flush_timers so that timing adds no host syncs.
+struct Pending {
+  std::string name;
+  double flops;
+  hipEvent_t a, b;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t s = nullptr;
+  std::unique_ptr<Pool> pool;
+  int o = 0, v = 0, naux = 0;
+  bool have_df = false, prepared = false;
+  Buf eo, ev, Boo, Bov, Bvv;
+  // 4-index blocks (Julia index order, column-major)
+  Buf OOOO, OOOV, OOVV, OVOV, OVVV, Lovov, Vperm, Pt;
+  // device-resident amplitudes (last guess / solve)
+  Buf T1, T2;
+  bool have_amp = false;
+  // scratch for reductions
+  Buf partials, scal;
+  std::map<std::string, Phase> phases;
+  std::vector<Pending> pending;
+  std::vector<IterRow> iters;
+  double last_energy = 0.0, last_et = 0.0;
+
+  int64_t nov() const { return (int64_t)o * v; }
+  int64_t n2() const { return (int64_t)o * o * v * v; }
+};
+
+// RAII span timer (see Pending)
+struct Timer {
+  Ctx& c;
+  Pending p;
+  Timer(Ctx& cc, const std::string& n, double f = 0.0) : c(cc) {
+    p.name = n;
+    p.flops = f;
+    FH_HIP(hipEventCreate(&p.a));
+    FH_HIP(hipEventCreate(&p.b));
+    FH_HIP(hipEventRecord(p.a, c.s));
+  }
+  ~Timer();
+};
+
+Timer::~Timer() {
+  if (hipEventRecord(p.b, c.s) == hipSuccess) c.pending.push_back(p);
+  else { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+}
+
+// resolve recorded spans (call after the stream has been synchronised)
+static void flush_timers(Ctx& c) {
+  for (auto& p : c.pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      Phase& ph = c.phases[p.name];
+      ph.ms += ms;
+      ph.flops += p.flops;
+      ph.calls += 1;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  c.pending.clear();
+}
+
+static Buf dalloc(Ctx& c, int64_t n) { return Buf(c.pool.get(), (size_t)std::max<int64_t>(n, 1) * sizeof(double)); }
+static void h2d(Ctx& c, double* d, const double* h, int64_t n) {
+  FH_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c.s));
+}
+static void d2h(Ctx& c, double* h, const double* d, int64_t n) {
+  FH_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c.s));
+  FH_HIP(hipStreamSynchronize(c.s));
+}
+static void d2d(Ctx& c, double* d, const double* s, int64_t n) {
+  FH_HIP(hipMemcpyAsync(d, s, n * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+}
+static double read_scalar(Ctx& c, const double* d) {
+  double h = 0.0;
+  FH_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, c.s));
+  FH_HIP(hipStreamSynchronize(c.s));
+  return h;
+}
+
+// ------------------------------------------------------------------ contraction engine
+// C[ic] = alpha * sum A[ia] B[ib] + beta * C[ic]  (einsum-style single-letter indices,
+// column-major tensors in the index order of the strings).  Evaluated as
+// permute -> MFMA GEMM -> permute, skipping every permute whose operand is already in a
+// GEMM-ready layout (including transposed operands).
+struct Letters {
+  std::string s;
+  const TView* t;
+  int64_t dim(char ch) const {
+    auto p = s.find(ch);
+    return t->dim[p];
+  }
+};
+
+static int64_t prod_dims(const Letters& L, const std::string& sub) {
+  int64_t n = 1;
+  for (char ch : sub) n *= L.dim(ch);
+  return n;
+}
+
+// input strides of `from` (column-major) listed in the order of `to`
+static void strides_for(const TView& from, const std::string& fromL, const std::string& toL, int64_t* istr,
+                        int64_t* odim) {
+  int64_t st[6];
+  int64_t acc = 1;
+  for (int d = 0; d < from.nd; ++d) { st[d] = acc; acc *= from.dim[d]; }
+  for (size_t d = 0; d < toL.size(); ++d) {
+    auto p = fromL.find(toL[d]);
+    istr[d] = st[p];
+    odim[d] = from.dim[p];
+  }
+}
+
+static void permute_to(Ctx& c, const TView& in, const std::string& inL, double* out, const std::string& outL,
+                       double alpha, double beta) {
+  int64_t istr[6], odim[6];
+  strides_for(in, inL, outL, istr, odim);
+  permute_f64(c.s, (int)outL.size(), odim, istr, alpha, in.p, beta, out);
+}
+
+static bool is_concat(const std::string& full, const std::string& first, const std::string& second) {
+  return full == first + second;
+}
+
+static void contract(Ctx& c, double alpha, const TView& A, const std::string& ia, const TView& B,
+                     const std::string& ib, double beta, const TView& C, const std::string& ic) {
+  FH_CHECK((int)ia.size() == A.nd && (int)ib.size() == B.nd && (int)ic.size() == C.nd, -3,
+           "contract: index string rank mismatch " + ia + "," + ib + "->" + ic);
+  Letters LA{ia, &A}, LB{ib, &B}, LC{ic, &C};
+  std::string M, N, K;
+  for (char ch : ic) {
+    const bool inA = ia.find(ch) != std::string::npos, inB = ib.find(ch) != std::string::npos;
+    FH_CHECK(inA != inB, -3, std::string("contract: output index in both/neither operand: ") + ch);
+    if (inA) M += ch; else N += ch;
+    FH_CHECK((inA ? LA.dim(ch) : LB.dim(ch)) == LC.dim(ch), -3, std::string("contract: size mismatch for ") + ch);
+  }
+  for (char ch : ia)
+    if (ic.find(ch) == std::string::npos) {
+      FH_CHECK(ib.find(ch) != std::string::npos, -3, std::string("contract: trace index unsupported: ") + ch);
+      FH_CHECK(LA.dim(ch) == LB.dim(ch), -3, std::string("contract: size mismatch for ") + ch);
+      K += ch;
+    }
+  for (char ch : ib)
+    FH_CHECK(ic.find(ch) != std::string::npos || ia.find(ch) != std::string::npos, -3,
+             std::string("contract: trace index unsupported: ") + ch);
+
+  // output handling: direct (C = M++N), swapped (C = N++M) or via a temporary
+  int cmode = 0;  // 0 direct, 1 swapped, 2 temp
+  auto sorted = [](std::string x) { std::sort(x.begin(), x.end()); return x; };
+  {
+    const std::string c1 = ic.substr(0, M.size()), c2 = ic.substr(M.size());
+    const std::string d1 = ic.substr(0, N.size()), d2 = ic.substr(N.size());
+    if (sorted(c1) == sorted(M) && sorted(c2) == sorted(N)) { cmode = 0; M = c1; N = c2; }
+    else if (sorted(d1) == sorted(N) && sorted(d2) == sorted(M)) { cmode = 1; N = d1; M = d2; }
+    else {
+      cmode = 2;
+      // keep operand orders so that A/B may be used as stored
+      std::string m2, n2;
+      for (char ch : ia) if (M.find(ch) != std::string::npos) m2 += ch;
+      for (char ch : ib) if (N.find(ch) != std::string::npos) n2 += ch;
+      M = m2; N = n2;
+    }
+  }
+  // choose the K order so that as many operands as possible are GEMM-ready
+  std::string KA, KB;
+  for (char ch : ia) if (K.find(ch) != std::string::npos) KA += ch;
+  for (char ch : ib) if (K.find(ch) != std::string::npos) KB += ch;
+  auto fitsA = [&](const std::string& k) { return is_concat(ia, M, k) || is_concat(ia, k, M); };
+  auto fitsB = [&](const std::string& k) { return is_concat(ib, k, N) || is_concat(ib, N, k); };
+  std::string Kord = KA;
+  if (!(fitsA(KA) && fitsB(KA)) && fitsA(KB) && fitsB(KB)) Kord = KB;
+  else if (!fitsA(KA) && fitsB(KB)) Kord = KB;
+
+  int64_t Msz = 1, Nsz = 1, Ksz = 1;
+  for (char ch : M) Msz *= LA.dim(ch);
+  for (char ch : N) Nsz *= LB.dim(ch);
+  for (char ch : Kord) Ksz *= LA.dim(ch);
+
+  // A as logical Msz x Ksz
+  Buf tA, tB, tC;
+  const double* pA = A.p;
+  bool transA = false;
+  int64_t lda;
+  if (is_concat(ia, M, Kord)) { lda = std::max<int64_t>(Msz, 1); }
+  else if (is_concat(ia, Kord, M)) { transA = true; lda = std::max<int64_t>(Ksz, 1); }
+  else {
+    tA = dalloc(c, A.size());
+    permute_to(c, A, ia, tA.d(), M + Kord, 1.0, 0.0);
+    pA = tA.d();
+    lda = std::max<int64_t>(Msz, 1);
+  }
+  const double* pB = B.p;
+  bool transB = false;
+  int64_t ldb;
+  if (is_concat(ib, Kord, N)) { ldb = std::max<int64_t>(Ksz, 1); }
+  else if (is_concat(ib, N, Kord)) { transB = true; ldb = std::max<int64_t>(Nsz, 1); }
+  else {
+    tB = dalloc(c, B.size());
+    permute_to(c, B, ib, tB.d(), Kord + N, 1.0, 0.0);
+    pB = tB.d();
+    ldb = std::max<int64_t>(Ksz, 1);
+  }
+  if (cmode == 0) {
+    gemm_f64(c.s, transA, transB, Msz, Nsz, Ksz, alpha, pA, lda, 0, pB, ldb, 0, beta, C.p, Msz, 0, C.p, Msz, 0);
+  } else if (cmode == 1) {
+    // C^T (Nsz x Msz) = op(B)^T op(A)^T
+    gemm_f64(c.s, !transB, !transA, Nsz, Msz, Ksz, alpha, pB, ldb, 0, pA, lda, 0, beta, C.p, Nsz, 0, C.p, Nsz, 0);
+  } else {
+    tC = dalloc(c, Msz * Nsz);
+    gemm_f64(c.s, transA, transB, Msz, Nsz, Ksz, 1.0, pA, lda, 0, pB, ldb, 0, 0.0, nullptr, Msz, 0, tC.d(), Msz, 0);
+    TView T;
+    T.p = tC.d();
+    T.nd = (int)(M.size() + N.size());
+    int q = 0;
+    for (char ch : M) T.dim[q++] = LA.dim(ch);
+    for (char ch : N) T.dim[q++] = LB.dim(ch);
+    permute_to(c, T, M + N, C.p, ic, alpha, beta);
+  }
+}
+
+// out[outL] = alpha * in[inL] + beta * out
+static void perm(Ctx& c, const TView& in, const std::string& inL, const TView& out, const std::string& outL,
+                 double alpha, double beta) {
+  FH_CHECK((int)inL.size() == in.nd && (int)outL.size() == out.nd, -3, "perm: rank mismatch");
+  permute_to(c, in, inL, out.p, outL, alpha, beta);
+}
+
+// ------------------------------------------------------------------ integrals
+static void require_df(Ctx& c) { FH_CHECK(c.have_df, -6, "no DF tensors: call fh_set_df first"); }
+
+static void prepare(Ctx& c) {
+  require_df(c);
+  if (c.prepared) return;
+  const int64_t o = c.o, v = c.v, Q = c.naux;
+  Timer t(c, "integrals", 2.0 * Q * (double)(o * o * o * o + o * o * o * v + o * o * v * v + o * v * o * v +
+                                              o * v * v * v + v * v * v * v));
+  TView Boo = tv(c.Boo.d(), {Q, o, o}), Bov = tv(c.Bov.d(), {Q, o, v}), Bvv = tv(c.Bvv.d(), {Q, v, v});
+  c.OOOO = dalloc(c, o * o * o * o);
+  c.OOOV = dalloc(c, o * o * o * v);
+  c.OOVV = dalloc(c, o * o * v * v);
+  c.OVOV = dalloc(c, o * v * o * v);
+  c.OVVV = dalloc(c, o * v * v * v);
+  c.Lovov = dalloc(c, o * v * o * v);
+  // DFERI.jl:83,109,134,151,177
+  contract(c, 1.0, Boo, "Qij", Boo, "Qkl", 0.0, tv(c.OOOO.d(), {o, o, o, o}), "ijkl");
+  contract(c, 1.0, Boo, "Qij", Bov, "Qka", 0.0, tv(c.OOOV.d(), {o, o, o, v}), "ijka");
+  contract(c, 1.0, Boo, "Qij", Bvv, "Qab", 0.0, tv(c.OOVV.d(), {o, o, v, v}), "ijab");
+  contract(c, 1.0, Bov, "Qia", Bov, "Qjb", 0.0, tv(c.OVOV.d(), {o, v, o, v}), "iajb");
+  contract(c, 1.0, Bov, "Qia", Bvv, "Qbc", 0.0, tv(c.OVVV.d(), {o, v, v, v}), "iabc");
+  // L[k,c,l,d] = 2 (kc|ld) - (kd|lc)
+  TView ovov = tv(c.OVOV.d(), {o, v, o, v});
+  perm(c, ovov, "kcld", tv(c.Lovov.d(), {o, v, o, v}), "kcld", 2.0, 0.0);
+  perm(c, ovov, "kdlc", tv(c.Lovov.d(), {o, v, o, v}), "kcld", -1.0, 1.0);
+  // ladder operand Vperm[c,d,a,b] = (ca|db) = Bvv[Q,c,a] Bvv[Q,d,b]  (DFERI.jl:194 permuted)
+  c.Vperm = dalloc(c, v * v * v * v);
+  contract(c, 1.0, Bvv, "Qca", Bvv, "Qdb", 0.0, tv(c.Vperm.d(), {v, v, v, v}), "cdab");
+  // Pt[c,d,a,k] = (kd|ac) = OVVV[k,d,a,c]
+  c.Pt = dalloc(c, v * v * v * o);
+  perm(c, tv(c.OVVV.d(), {o, v, v, v}), "kdac", tv(c.Pt.d(), {v, v, v, o}), "cdak", 1.0, 0.0);
+  c.prepared = true;
+}
+
+// ------------------------------------------------------------------ CCSD numerators
+// R1 = numerator of the T1 update (RCCSDHelper.jl:57-85); R2 = numerator of the T2 update
+// (RCCSDHelper.jl:123-220), both without the diagonal Fock term.
+static void residual(Ctx& c, const double* t1p, const double* t2p, double* R1, double* R2) {
+  prepare(c);
+  const int64_t o = c.o, v = c.v;
+  const double o_ = (double)o, v_ = (double)v;
+  Timer tm(c, "residual", 0.0);
+  TView t1 = tv(const_cast<double*>(t1p), {o, v}), t2 = tv(const_cast<double*>(t2p), {o, o, v, v});
+  TView ovov = tv(c.OVOV.d(), {o, v, o, v}), oooo = tv(c.OOOO.d(), {o, o, o, o}), oovv = tv(c.OOVV.d(), {o, o, v, v});
+  TView ovvv = tv(c.OVVV.d(), {o, v, v, v}), ooov = tv(c.OOOV.d(), {o, o, o, v}), Lovov = tv(c.Lovov.d(), {o, v, o, v});
+  TView r1 = tv(R1, {o, v});
+
+  Buf bFov = dalloc(c, o * v), bFoo = dalloc(c, o * o), bFvv = dalloc(c, v * v), bX = dalloc(c, o * o);
+  TView Fov = tv(bFov.d(), {o, v}), Foo = tv(bFoo.d(), {o, o}), Fvv = tv(bFvv.d(), {v, v}), X = tv(bX.d(), {o, o});
+  Buf bu = dalloc(c, c.n2()), btau = dalloc(c, c.n2());
+  TView u = tv(bu.d(), {o, o, v, v}), tau = tv(btau.d(), {o, o, v, v});
+  // u[i,j,a,b] = 2 t2[i,j,a,b] - t2[i,j,b,a];  tau = t2 + t1 (x) t1
+  perm(c, t2, "ijab", u, "ijab", 2.0, 0.0);
+  perm(c, t2, "ijba", u, "ijab", -1.0, 1.0);
+  tau_f64(c.s, (int)o, (int)v, t1p, t2p, 1.0, tau.p);
+
+  // Fov[k,c] = L[k,c,l,d] t1[l,d];  Foo[k,i] = L[k,c,l,d] t2[i,l,c,d] + Fov[k,c] t1[i,c];
+  // Fvv[a,c] = -L[k,c,l,d] t2[k,l,a,d] - t1[k,a] Fov[k,c]
+  contract(c, 1.0, Lovov, "kcld", t1, "ld", 0.0, Fov, "kc");
+  contract(c, 1.0, Lovov, "kcld", t2, "ilcd", 0.0, Foo, "ki");
+  contract(c, 1.0, Fov, "kc", t1, "ic", 1.0, Foo, "ki");
+  contract(c, -1.0, Lovov, "kcld", t2, "klad", 0.0, Fvv, "ac");
+  contract(c, -1.0, t1, "ka", Fov, "kc", 1.0, Fvv, "ac");
+  // Loo = Foo + 2 (ki|lc) t1[l,c] - (li|kc) t1[l,c];  Lvv = Fvv + 2 (kd|ac) t1[k,d] - (kc|ad) t1[k,d]
+  Buf bLoo = dalloc(c, o * o), bLvv = dalloc(c, v * v);
+  TView Loo = tv(bLoo.d(), {o, o}), Lvv = tv(bLvv.d(), {v, v});
+  d2d(c, Loo.p, Foo.p, o * o);
+  d2d(c, Lvv.p, Fvv.p, v * v);
+  contract(c, 2.0, ooov, "kilc", t1, "lc", 1.0, Loo, "ki");
+  contract(c, -1.0, ooov, "likc", t1, "lc", 1.0, Loo, "ki");
+  contract(c, 2.0, ovvv, "kdac", t1, "kd", 1.0, Lvv, "ac");
+  contract(c, -1.0, ovvv, "kcad", t1, "kd", 1.0, Lvv, "ac");
+
+  // ---- T1 numerator
+  contract(c, 1.0, Lvv, "ac", t1, "ic", 0.0, r1, "ia");
+  contract(c, -1.0, Loo, "ki", t1, "ka", 1.0, r1, "ia");
+  contract(c, 1.0, Fov, "kc", u, "kica", 1.0, r1, "ia");
+  contract(c, 1.0, Fov, "kc", t1, "ic", 0.0, X, "ik");
+  contract(c, 1.0, X, "ik", t1, "ka", 1.0, r1, "ia");
+  contract(c, 2.0, ovov, "kcia", t1, "kc", 1.0, r1, "ia");
+  contract(c, -1.0, oovv, "kiac", t1, "kc", 1.0, r1, "ia");
+  contract(c, 1.0, ovvv, "kdac", u, "ikcd", 1.0, r1, "ia");
+  contract(c, -1.0, ooov, "kilc", u, "klac", 1.0, r1, "ia");
+
+  // ---- T2 numerator: P, then R2 = P + P[j,i,b,a]
+  Buf bP = dalloc(c, c.n2());
+  TView P = tv(bP.d(), {o, o, v, v});
+  perm(c, ovov, "iajb", P, "ijab", 0.5, 0.0);                       // (ia|jb)  (:134)
+  contract(c, 1.0, ovvv, "iacb", t1, "jc", 1.0, P, "ijab");          // T1 x (ia|cb)
+  {
+    Buf b1 = dalloc(c, o * o * v * o);
+    TView X1 = tv(b1.d(), {o, o, v, o});
+    contract(c, 1.0, oovv, "kibc", t1, "jc", 0.0, X1, "kibj");
+    contract(c, -1.0, X1, "kibj", t1, "ka", 1.0, P, "ijab");
+    Buf b2 = dalloc(c, v * o * o * o);
+    TView X2 = tv(b2.d(), {v, o, o, o});
+    contract(c, 1.0, ovov, "kcia", t1, "jc", 0.0, X2, "akij");
+    perm(c, ooov, "jkia", X2, "akij", 1.0, 1.0);
+    contract(c, -1.0, X2, "akij", t1, "kb", 1.0, P, "ijab");
+  }
+  {
+    // Woooo[k,l,i,j] = (ki|lj) + (ki|lc) t1[j,c] + (lj|kc) t1[i,c] + (kc|ld) tau[i,j,c,d]
+    Buf bw = dalloc(c, o * o * o * o);
+    TView W = tv(bw.d(), {o, o, o, o});
+    perm(c, oooo, "kilj", W, "klij", 1.0, 0.0);
+    contract(c, 1.0, ooov, "kilc", t1, "jc", 1.0, W, "klij");
+    contract(c, 1.0, ooov, "ljkc", t1, "ic", 1.0, W, "klij");
+    contract(c, 1.0, ovov, "kcld", tau, "ijcd", 1.0, W, "klij");
+    contract(c, 0.5, W, "klij", tau, "klab", 1.0, P, "ijab");
+  }
+  {
+    // particle-particle ladder: P += tau[i,j,c,d] * Y[c,d,a,b],
+    // Y = 0.5*(ca|db) - sum_k (kd|ac) t1[k,b]   (Wvvvv with the symmetric half folded into P)
+    Buf bY = dalloc(c, v * v * v * v);
+    {
+      Timer tl(c, "ladder_build", 2.0 * o_ * v_ * v_ * v_ * v_);
+      gemm_f64(c.s, false, false, v * v * v, v, o, -1.0, c.Pt.d(), v * v * v, 0, t1p, o, 0, 0.5, c.Vperm.d(),
+               v * v * v, 0, bY.d(), v * v * v, 0);
+    }
+    Timer tl(c, "ladder", 2.0 * o_ * o_ * v_ * v_ * v_ * v_);
+    gemm_f64(c.s, false, false, o * o, v * v, v * v, 1.0, tau.p, o * o, 0, bY.d(), v * v, 0, 1.0, P.p, o * o, 0, P.p,
+             o * o, 0, 1, /*tag: PP ladder*/ 1);
+  }
+  contract(c, 1.0, Lvv, "ac", t2, "ijcb", 1.0, P, "ijab");
+  contract(c, -1.0, Loo, "ki", t2, "kjab", 1.0, P, "ijab");
+  {
+    Timer tr(c, "ring", 6.0 * 2.0 * o_ * o_ * o_ * v_ * v_ * v_);
+    Buf bWa = dalloc(c, c.n2()), bWb = dalloc(c, c.n2()), bXX = dalloc(c, c.n2());
+    TView Wvoov = tv(bWa.d(), {v, o, o, v}), Wvovo = tv(bWb.d(), {v, o, v, o});
+    // Wvoov[a,k,i,c]
+    perm(c, ovov, "kcia", Wvoov, "akic", 1.0, 0.0);
+    contract(c, 1.0, ovvv, "kcad", t1, "id", 1.0, Wvoov, "akic");
+    contract(c, -1.0, ooov, "likc", t1, "la", 1.0, Wvoov, "akic");
+    TView XX = tv(bXX.d(), {o, o, v, o});
+    contract(c, 1.0, ovov, "ldkc", t1, "id", 0.0, XX, "lkci");
+    contract(c, -1.0, XX, "lkci", t1, "la", 1.0, Wvoov, "akic");
+    contract(c, 0.5, ovov, "ldkc", u, "ilad", 1.0, Wvoov, "akic");
+    contract(c, -0.5, ovov, "lckd", t2, "ilad", 1.0, Wvoov, "akic");
+    // Wvovo[a,k,c,i]
+    perm(c, oovv, "kiac", Wvovo, "akci", 1.0, 0.0);
+    contract(c, 1.0, ovvv, "kdac", t1, "id", 1.0, Wvovo, "akci");
+    contract(c, -1.0, ooov, "kilc", t1, "la", 1.0, Wvovo, "akci");
+    TView XY = tv(bXX.d(), {o, v, o, o});
+    contract(c, 1.0, ovov, "lckd", t1, "id", 0.0, XY, "lcki");
+    contract(c, -1.0, XY, "lcki", t1, "la", 1.0, Wvovo, "akci");
+    contract(c, -0.5, ovov, "lckd", t2, "ilda", 1.0, Wvovo, "akci");
+    // ring / crossed-ring application
+    contract(c, 1.0, Wvoov, "akic", u, "kjcb", 1.0, P, "ijab");
+    contract(c, -1.0, Wvovo, "akci", t2, "kjcb", 1.0, P, "ijab");
+    contract(c, -1.0, Wvovo, "bkci", t2, "kjac", 1.0, P, "ijab");
+  }
+  TView r2 = tv(R2, {o, o, v, v});
+  perm(c, P, "ijab", r2, "ijab", 1.0, 0.0);
+  perm(c, P, "jiba", r2, "ijab", 1.0, 1.0);                          // (:190)
+}
+
+static double energy(Ctx& c, const double* t1, const double* t2) {
+  prepare(c);
+  if (!c.partials.p) c.partials = dalloc(c, 4096);
+  if (!c.scal.p) c.scal = dalloc(c, 64);
+  cc_energy(c.s, c.o, c.v, t1, t2, c.OVOV.d(), c.partials.d(), c.scal.d());
+  return read_scalar(c, c.scal.d());
+}
+
+// newT = residual(T) ./ D; returns rms of (newT1 - T1), (newT2 - T2); errors optionally kept
+static void update_amp(Ctx& c, const double* t1, const double* t2, double* n1, double* n2, void* e1, void* e2,
+                       bool f32, double* r1, double* r2) {
+  residual(c, t1, t2, n1, n2);
+  if (!c.partials.p) c.partials = dalloc(c, 4096);
+  if (!c.scal.p) c.scal = dalloc(c, 64);
+  int64_t nb = 0;
+  jacobi_update(c.s, c.o, c.v, false, c.eo.d(), c.ev.d(), n1, t1, e1, f32, c.partials.d(), &nb);
+  sum_partials(c.s, nb, c.partials.d(), c.scal.d());
+  jacobi_update(c.s, c.o, c.v, true, c.eo.d(), c.ev.d(), n2, t2, e2, f32, c.partials.d() + 2048, &nb);
+  sum_partials(c.s, nb, c.partials.d() + 2048, c.scal.d() + 1);
+  double h[2];
+  FH_HIP(hipMemcpyAsync(h, c.scal.d(), 2 * sizeof(double), hipMemcpyDeviceToHost, c.s));
+  FH_HIP(hipStreamSynchronize(c.s));
+  *r1 = std::sqrt(h[0] / (double)c.nov());
+  *r2 = std::sqrt(h[1] / (double)c.n2());
+}
+
+static void mp2_guess(Ctx& c) {
+  prepare(c);
+  const int64_t o = c.o, v = c.v;
+  if (!c.T1.p) c.T1 = dalloc(c, o * v);
+  if (!c.T2.p) c.T2 = dalloc(c, c.n2());
+  fill_f64(c.s, o * v, 0.0, c.T1.d());  // Fia = 0 for RHF orbitals (Fock.jl:95)
+  perm(c, tv(c.OVOV.d(), {o, v, o, v}), "iajb", tv(c.T2.d(), {o, o, v, v}), "ijab", 1.0, 0.0);
+  if (!c.partials.p) c.partials = dalloc(c, 4096);
+  int64_t nb = 0;
+  Buf zero = dalloc(c, c.n2());
+  fill_f64(c.s, c.n2(), 0.0, zero.d());
+  jacobi_update(c.s, c.o, c.v, true, c.eo.d(), c.ev.d(), c.T2.d(), zero.d(), nullptr, false, c.partials.d(), &nb);
+  c.have_amp = true;
+}
+
+// ------------------------------------------------------------------ DIIS (DIIS.jl:21-96)
+struct DIIS {
+  Ctx& c;
+  int64_t n;
+  int maxv;
+  bool f32;
+  std::vector<Buf> vecs, errs;
+  DIIS(Ctx& cc, int64_t nn, int m, bool f) : c(cc), n(nn), maxv(m), f32(f) {}
+  double dot(const Buf& a, const Buf& b) {
+    if (f32) dot_f32(c.s, n, a.f(), b.f(), c.partials.d(), c.scal.d() + 4);
+    else dot_f64(c.s, n, a.d(), b.d(), c.partials.d(), c.scal.d() + 4);
+    return read_scalar(c, c.scal.d() + 4);
+  }
+  // push! takes ownership of the error buffer and copies the vector
+  void push(const double* vec, Buf&& err) {
+    if ((int)vecs.size() + 1 > maxv) {
+      int idx = 0;
+      double best = -1.0;
+      for (size_t q = 0; q < errs.size(); ++q) {
+        const double nq = std::sqrt(dot(errs[q], errs[q]));
+        if (nq > best) { best = nq; idx = (int)q; }   // findmax: first maximum
+      }
+      vecs.erase(vecs.begin() + idx);
+      errs.erase(errs.begin() + idx);
+    }
+    Buf vb = dalloc(c, n);
+    d2d(c, vb.d(), vec, n);
+    vecs.push_back(std::move(vb));
+    errs.push_back(std::move(err));
+  }
+  void extrapolate(double* out, bool add_res) {
+    const int m = (int)vecs.size();
+    FH_CHECK(m >= 2, -3, "DIIS cannot extrapolate from one vector");
+    const int N = m + 1;
+    std::vector<double> B(N * N, 1.0), rhs(N, 0.0);
+    B[(N - 1) * N + (N - 1)] = 0.0;
+    for (int a = 0; a < m; ++a)
+      for (int b = 0; b <= a; ++b) {
+        const double d = dot(errs[a], errs[b]);
+        B[a * N + b] = d;
+        B[b * N + a] = d;
+      }
+    rhs[N - 1] = 1.0;
+    // B \ rhs by LU with partial pivoting (row-major)
+    std::vector<double> A = B, x = rhs;
+    for (int k = 0; k < N; ++k) {
+      int p = k;
+      for (int r = k + 1; r < N; ++r)
+        if (std::fabs(A[r * N + k]) > std::fabs(A[p * N + k])) p = r;
+      if (p != k) {
+        for (int cc = 0; cc < N; ++cc) std::swap(A[k * N + cc], A[p * N + cc]);
+        std::swap(x[k], x[p]);
+      }
+      FH_CHECK(A[k * N + k] != 0.0, -5, "DIIS: singular B matrix");
+      for (int r = k + 1; r < N; ++r) {
+        const double f = A[r * N + k] / A[k * N + k];
+        for (int cc = k; cc < N; ++cc) A[r * N + cc] -= f * A[k * N + cc];
+        x[r] -= f * x[k];
+      }
+    }
+    for (int k = N - 1; k >= 0; --k) {
+      double sacc = x[k];
+      for (int cc = k + 1; cc < N; ++cc) sacc -= A[k * N + cc] * x[cc];
+      x[k] = sacc / A[k * N + k];
+    }
+    std::vector<const double*> xs(m);
+    std::vector<const void*> es(m);
+    for (int q = 0; q < m; ++q) { xs[q] = vecs[q].d(); es[q] = errs[q].p; }
+    lincomb(c.s, n, m, xs.data(), x.data(), es.data(), f32, add_res, out);
+  }
+};
+
+static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
+  prepare(c);
+  FH_CHECK(c.have_amp, -6, "no starting amplitudes");
+  FH_CHECK(op.cc_max_iter >= 0 && op.cc_ndiis >= 1 && op.cc_ndiis <= 8 && op.cc_diis_relax >= 1, -1,
+           "invalid CC options");
+  const int64_t n1 = c.nov(), n2 = c.n2();
+  const double epsT = 2.220446049250313e-16;
+  double e_conv = op.cc_e_conv, max_rms = op.cc_max_rms;
+  if ((e_conv < epsT || max_rms < epsT) && !op.precision_override) e_conv = max_rms = epsT;   // RCCSDa.jl:139-146
+  const bool do_diis = op.cc_diis != 0;
+  const bool f32 = op.diis_prec != 0;
+  const double dp = op.cc_damp_ratio;
+  DIIS dm1(c, n1, op.cc_ndiis, f32), dm2(c, n2, op.cc_ndiis, f32);
+  c.iters.clear();
+
+  Buf nT1 = std::move(c.T1), nT2 = std::move(c.T2);     // newT (guess)
+  Buf oT1 = dalloc(c, n1), oT2 = dalloc(c, n2);         // T
+  double Ecc = energy(c, nT1.d(), nT2.d());            // :160
+  res.guess_energy = Ecc;
+  double dE = 1.0, rms = 1.0;
+  int ite = 1;
+  if (op.verbose) std::printf("%10s    %15s    %12s    %12s    %10s    %5s\n", "Iteration", "CC Energy", "dE", "Max RMS", "time (s)", "diis");
+  while (std::fabs(dE) > e_conv || rms > max_rms) {                                             // :178
+    if (ite > op.cc_max_iter) break;
+    auto t0 = std::chrono::steady_clock::now();
+    std::swap(oT1, nT1);                                                                        // :185-186
+    std::swap(oT2, nT2);
+    if (!nT1.p) nT1 = dalloc(c, n1);
+    if (!nT2.p) nT2 = dalloc(c, n2);
+    const double oldE = Ecc;
+    Buf e1, e2;
+    if (do_diis) {
+      e1 = Buf(c.pool.get(), (size_t)n1 * (f32 ? 4 : 8));
+      e2 = Buf(c.pool.get(), (size_t)n2 * (f32 ? 4 : 8));
+    }
+    double r1 = 0.0, r2 = 0.0;
+    update_amp(c, oT1.d(), oT2.d(), nT1.d(), nT2.d(), e1.p, e2.p, f32, &r1, &r2);               // :190-194
+    bool diis = false;
+    if (do_diis) {
+      dm1.push(nT1.d(), std::move(e1));                                                         // :199-202
+      dm2.push(nT2.d(), std::move(e2));
+      if (ite >= op.diis_start && ite % op.cc_diis_relax == 0) {                                // :204
+        dm2.extrapolate(nT2.d(), true);                                                         // :206
+        dm1.extrapolate(nT1.d(), true);                                                         // :207
+        diis = true;
+      }
+    }
+    if (dp > 0) {                                                                               // :213-216
+      axpby_f64(c.s, n1, dp, oT1.d(), 1.0 - dp, nT1.d());
+      axpby_f64(c.s, n2, dp, oT2.d(), 1.0 - dp, nT2.d());
+    }
+    Ecc = energy(c, nT1.d(), nT2.d());                                                          // :218
+    rms = std::max(r1, r2);
+    dE = Ecc - oldE;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c.iters.push_back({ite, Ecc, dE, rms, ms, diis});
+    if (op.verbose) std::printf("    %-5d    %-15.10f    %+12.10f    %-12.10f    %-10.5f    %s\n", ite, Ecc, dE, rms, ms / 1e3, diis ? "true" : "false");
+    FH_CHECK(std::isfinite(Ecc) && std::isfinite(rms), -5, "non-finite CC energy or amplitudes at iteration " + std::to_string(ite));
+    ++ite;
+  }
+  res.correlation = Ecc;
+  res.e_conv = dE;
+  res.t_conv = rms;
+  res.niter = ite - 1;
+  res.converged = (std::fabs(dE) < e_conv && rms < max_rms) ? 1 : 0;                          // :231
+  c.T1 = std::move(nT1);
+  c.T2 = std::move(nT2);
+  c.have_amp = true;
+  c.last_energy = Ecc;
+}
+
+// ------------------------------------------------------------------ (T), ijk.jl:20-150
+__global__ void k_build_av(int o, int v, const double* __restrict__ ovvv, const double* __restrict__ t2,
+                           double* __restrict__ av) {
+  // AV[(x,y), n, d'] : d' < v  -> (n y | x d') = OVVV[n, y, x, d']   (ijk2.jl:33 vvvo layout)
+  //                    d' >= v -> -T2[n, l, y, x], l = d' - v       (T2_n with the GEMM's minus sign)
+  const int64_t vv = (int64_t)v * v, nd = (int64_t)v + o;
+  const int64_t total = vv * o * nd;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+    const int64_t x = idx % v, y = (idx / v) % v, n = (idx / vv) % o, dp = idx / (vv * o);
+    double val;
+    if (dp < v) val = ovvv[n + (int64_t)o * (y + (int64_t)v * (x + (int64_t)v * dp))];
+    else {
+      const int64_t l = dp - v;
+      val = -t2[n + (int64_t)o * (l + (int64_t)o * (y + (int64_t)v * x))];
+    }
+    av[idx] = val;
+  }
+}
+__global__ void k_build_bv(int o, int v, bool swap_pq, const double* __restrict__ t2, const double* __restrict__ ooov,
+                           double* __restrict__ bv) {
+  // Bcat_{p,q}[d', z]: d' < v -> T2p[d', z, p, q] = T2[q, p, z, d'];  d' >= v -> ovoo[l, z, p, q] = OOOV[l, p, q, z]
+  // stored as bv[d', z, p, q] (swap_pq = false) or bv[d', z, q, p] (swap_pq = true)
+  const int64_t nd = (int64_t)v + o;
+  const int64_t total = nd * v * o * o;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+    const int64_t dp = idx % nd, z = (idx / nd) % v, r = (idx / (nd * v)) % o, s = idx / (nd * v * o);
+    const int64_t p = swap_pq ? s : r, q = swap_pq ? r : s;
+    double val;
+    if (dp < v) val = t2[q + (int64_t)o * (p + (int64_t)o * (z + (int64_t)v * dp))];
+    else {
+      const int64_t l = dp - v;
+      val = ooov[l + (int64_t)o * (p + (int64_t)o * (q + (int64_t)o * z))];
+    }
+    bv[idx] = val;
+  }
+}
+__global__ void k_accumulate(int64_t n, const double* __restrict__ partials, double* __restrict__ acc) {
+  // single block; fixed order -> deterministic
+  __shared__ double red[256];
+  double t = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) t += partials[i];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *acc += red[0];
+}
+
+static double triples(Ctx& c) {
+  prepare(c);
+  FH_CHECK(c.have_amp, -6, "no amplitudes for (T)");
+  const int64_t o = c.o, v = c.v, vv = v * v, nd = v + o;
+  const double o_ = (double)o, v_ = (double)v;
+  const double ntrip = o_ * (o_ + 1) * (o_ + 2) / 6.0;
+  Timer tall(c, "triples", ntrip * 12.0 * (v_ * v_ * v_ * v_ + o_ * v_ * v_ * v_));
+  Buf AV = dalloc(c, vv * o * nd), BVp = dalloc(c, nd * v * o * o), BVq = dalloc(c, nd * v * o * o);
+  Buf T1p = dalloc(c, v * o), vvoo = dalloc(c, vv * o * o);
+  {
+    auto g = [](int64_t n) { int64_t b = (n + 255) / 256; return (unsigned)std::min<int64_t>(b, 65536 * 4); };
+    hipLaunchKernelGGL(k_build_av, dim3(g(vv * o * nd)), dim3(256), 0, c.s, (int)o, (int)v, c.OVVV.d(), c.T2.d(), AV.d());
+    hipLaunchKernelGGL(k_build_bv, dim3(g(nd * v * o * o)), dim3(256), 0, c.s, (int)o, (int)v, false, c.T2.d(), c.OOOV.d(), BVp.d());
+    hipLaunchKernelGGL(k_build_bv, dim3(g(nd * v * o * o)), dim3(256), 0, c.s, (int)o, (int)v, true, c.T2.d(), c.OOOV.d(), BVq.d());
+    FH_HIP(hipGetLastError());
+    perm(c, tv(c.T1.d(), {o, v}), "ia", tv(T1p.d(), {v, o}), "ai", 1.0, 0.0);
+    perm(c, tv(c.OVOV.d(), {o, v, o, v}), "iajb", tv(vvoo.d(), {v, v, o, o}), "abij", 1.0, 0.0);
+  }
+  // k-chunk: bound the six class buffers to ~16 GB
+  int64_t KC = std::max<int64_t>(1, std::min<int64_t>(o, (int64_t)(16e9 / (6.0 * 8.0 * (double)vv * v))));
+  std::vector<Buf> G;
+  for (int q = 0; q < 6; ++q) G.push_back(dalloc(c, vv * v * KC));
+  const int64_t npair = v * (v + 1) / 2;
+  Buf part = dalloc(c, ((npair + 255) / 256) * KC + 16);
+  Buf acc = dalloc(c, 1);
+  fill_f64(c.s, 1, 0.0, acc.d());
+  const int64_t bstride = nd * v;  // one (p,q) column block of BVp/BVq
+  for (int64_t i = 0; i < o; ++i)
+    for (int64_t j = 0; j <= i; ++j) {
+      const int64_t kend = (i == j) ? j : j + 1;  // i=j=k has zero weight (ijk.jl:133)
+      for (int64_t k0 = 0; k0 < kend; k0 += KC) {
+        const int64_t nk = std::min<int64_t>(KC, kend - k0);
+        const double* AVj = AV.d() + vv * j;
+        const double* AVi = AV.d() + vv * i;
+        std::unique_ptr<Timer> tg(new Timer(c, "triples_gemm", 12.0 * (double)vv * (double)v * (double)nd * (double)nk));
+        // N-extended classes: G[(x,y),(z,k)] = AV_n (v^2 x nd) * B (nd x v*nk)
+        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVj, vv * o, 0, BVq.d() + bstride * (k0 + o * i), nd, 0, 0.0,
+                 nullptr, vv, 0, G[0].d(), vv, 0, 1, 2);   // abc: B_{i,k}
+        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVi, vv * o, 0, BVq.d() + bstride * (k0 + o * j), nd, 0, 0.0,
+                 nullptr, vv, 0, G[2].d(), vv, 0, 1, 2);   // bac: B_{j,k}
+        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVi, vv * o, 0, BVp.d() + bstride * (k0 + o * j), nd, 0, 0.0,
+                 nullptr, vv, 0, G[4].d(), vv, 0, 1, 2);   // cab: B_{k,j}
+        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVj, vv * o, 0, BVp.d() + bstride * (k0 + o * i), nd, 0, 0.0,
+                 nullptr, vv, 0, G[5].d(), vv, 0, 1, 2);   // cba: B_{k,i}
+        // M-extended classes: G[(x,y,k), z] = AV_k rows (v^2*nk x nd) * B_{p,q} (nd x v)
+        gemm_f64(c.s, false, false, vv * nk, v, nd, 1.0, AV.d() + vv * k0, vv * o, 0, BVp.d() + bstride * (i + o * j),
+                 nd, 0, 0.0, nullptr, vv * nk, 0, G[1].d(), vv * nk, 0, 1, 2);   // acb: B_{i,j}
+        gemm_f64(c.s, false, false, vv * nk, v, nd, 1.0, AV.d() + vv * k0, vv * o, 0, BVp.d() + bstride * (j + o * i),
+                 nd, 0, 0.0, nullptr, vv * nk, 0, G[3].d(), vv * nk, 0, 1, 2);   // bca: B_{j,i}
+        tg.reset();
+        Timer te(c, "triples_energy", 0.0);
+        TriplesDesc d;
+        for (int q = 0; q < 6; ++q) d.G[q] = G[q].d();
+        const int64_t v3 = vv * v, vk = vv * nk;
+        // abc: x=a y=b z=c      | bac: x=b y=a z=c      | cab: x=c y=a z=b      | cba: x=c y=b z=a
+        d.sa[0] = 1;  d.sb[0] = v;  d.sc[0] = vv; d.sk[0] = v3;
+        d.sa[2] = v;  d.sb[2] = 1;  d.sc[2] = vv; d.sk[2] = v3;
+        d.sa[4] = v;  d.sb[4] = vv; d.sc[4] = 1;  d.sk[4] = v3;
+        d.sa[5] = vv; d.sb[5] = v;  d.sc[5] = 1;  d.sk[5] = v3;
+        // acb: x=a y=c k z=b     | bca: x=b y=c k z=a
+        d.sa[1] = 1;  d.sc[1] = v;  d.sk[1] = vv; d.sb[1] = vk;
+        d.sb[3] = 1;  d.sc[3] = v;  d.sk[3] = vv; d.sa[3] = vk;
+        int64_t nb = 0;
+        triples_energy(c.s, (int)o, (int)v, (int)i, (int)j, (int)k0, (int)nk, d, T1p.d(), vvoo.d(), c.eo.d(), c.ev.d(),
+                       part.d(), &nb);
+        hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(256), 0, c.s, nb, part.d(), acc.d());
+        FH_HIP(hipGetLastError());
+      }
+    }
+  const double Et = read_scalar(c, acc.d());
+  c.last_et = Et;
+  return Et;
+}
+
+}  // namespace fh
+
+// ====================================================================== C ABI
+using namespace fh;
+
+struct fh_ctx {
+  Ctx c;
+};
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return -3;
+  }
+}
+
+struct DeviceGuard {
+  int prev = 0;
+  explicit DeviceGuard(int d) { (void)hipGetDevice(&prev); (void)hipSetDevice(d); }
+  ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+extern "C" {
+
+void fh_default_opts(fh_cc_opts* p) {
+  if (!p) return;
+  p->cc_max_iter = 50;
+  p->cc_e_conv = 1e-10;
+  p->cc_max_rms = 1e-10;
+  p->cc_diis = 1;
+  p->cc_ndiis = 3;
+  p->diis_start = 3;
+  p->cc_diis_relax = 3;
+  p->diis_prec = 1;
+  p->cc_damp_ratio = 0.0;
+  p->precision_override = 0;
+  p->verbose = 0;
+}
+
+const char* fh_last_error(void) { return g_last_error.c_str(); }
+
+int fh_create(fh_ctx** out, int device) {
+  return guarded([&] {
+    FH_CHECK(out != nullptr, -1, "fh_create: null output pointer");
+    *out = nullptr;
+    int n = 0;
+    FH_HIP(hipGetDeviceCount(&n));
+    FH_CHECK(device >= 0 && device < n, -1, "fh_create: no HIP device " + std::to_string(device));
+    DeviceGuard g(device);
+    auto* x = new fh_ctx();
+    x->c.device = device;
+    x->c.pool.reset(new Pool(device));
+    hipError_t e = hipStreamCreateWithFlags(&x->c.s, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete x; throw Error{-2, std::string("hipStreamCreate: ") + hipGetErrorString(e)}; }
+    *out = x;
+  });
+}
+
+void fh_destroy(fh_ctx* x) {
+  if (!x) return;
+  DeviceGuard g(x->c.device);
+  (void)hipStreamSynchronize(x->c.s);
+  flush_timers(x->c);
+  {
+    Ctx& c = x->c;
+    c.eo.reset(); c.ev.reset(); c.Boo.reset(); c.Bov.reset(); c.Bvv.reset();
+    c.OOOO.reset(); c.OOOV.reset(); c.OOVV.reset(); c.OVOV.reset(); c.OVVV.reset(); c.Lovov.reset();
+    c.Vperm.reset(); c.Pt.reset(); c.T1.reset(); c.T2.reset(); c.partials.reset(); c.scal.reset();
+  }
+  (void)hipStreamDestroy(x->c.s);
+  x->c.pool.reset();
+  delete x;
+}
+
+int fh_set_df(fh_ctx* x, int o, int v, int naux, const double* eps_o, const double* eps_v, const double* Boo,
+              const double* Bov, const double* Bvv) {
+  return guarded([&] {
+    FH_CHECK(x != nullptr, -1, "fh_set_df: null context");
+    FH_CHECK(o >= 1 && v >= 1 && naux >= 1, -1, "fh_set_df: invalid dimensions");
+    FH_CHECK(eps_o && eps_v && Boo && Bov && Bvv, -1, "fh_set_df: null array");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    for (int i = 0; i < o; ++i)
+      for (int a = 0; a < v; ++a)
+        FH_CHECK(eps_o[i] < eps_v[a], -1, "fh_set_df: occupied orbital energy above a virtual one");
+    c.prepared = false;
+    c.have_amp = false;
+    c.OOOO.reset(); c.OOOV.reset(); c.OOVV.reset(); c.OVOV.reset(); c.OVVV.reset(); c.Lovov.reset();
+    c.Vperm.reset(); c.Pt.reset(); c.T1.reset(); c.T2.reset();
+    c.o = o; c.v = v; c.naux = naux;
+    const int64_t Q = naux;
+    c.eo = dalloc(c, o); c.ev = dalloc(c, v);
+    c.Boo = dalloc(c, Q * o * o); c.Bov = dalloc(c, Q * o * v); c.Bvv = dalloc(c, Q * v * v);
+    h2d(c, c.eo.d(), eps_o, o);
+    h2d(c, c.ev.d(), eps_v, v);
+    h2d(c, c.Boo.d(), Boo, Q * o * o);
+    h2d(c, c.Bov.d(), Bov, Q * o * v);
+    h2d(c, c.Bvv.d(), Bvv, Q * v * v);
+    FH_HIP(hipStreamSynchronize(c.s));
+    c.have_df = true;
+  });
+}
+
+int fh_prepare(fh_ctx* x) {
+  return guarded([&] {
+    FH_CHECK(x != nullptr, -1, "null context");
+    DeviceGuard g(x->c.device);
+    prepare(x->c);
+    FH_HIP(hipStreamSynchronize(x->c.s));
+    flush_timers(x->c);
+  });
+}
+
+static void upload_amp(Ctx& c, const double* T1, const double* T2) {
+  if (!c.T1.p) c.T1 = dalloc(c, c.nov());
+  if (!c.T2.p) c.T2 = dalloc(c, c.n2());
+  h2d(c, c.T1.d(), T1, c.nov());
+  h2d(c, c.T2.d(), T2, c.n2());
+  c.have_amp = true;
+}
+
+int fh_t2_guess(fh_ctx* x, int kind, double* T1, double* T2) {
+  return guarded([&] {
+    FH_CHECK(x != nullptr, -1, "null context");
+    FH_CHECK(kind == 0, -1, "fh_t2_guess: only kind 0 (MP2) is available");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    require_df(c);
+    mp2_guess(c);
+    if (T1) d2h(c, T1, c.T1.d(), c.nov());
+    if (T2) d2h(c, T2, c.T2.d(), c.n2());
+    FH_HIP(hipStreamSynchronize(c.s));
+    flush_timers(c);
+  });
+}
+
+int fh_ccsd_energy(fh_ctx* x, const double* T1, const double* T2, double* E) {
+  return guarded([&] {
+    FH_CHECK(x != nullptr && E != nullptr, -1, "null argument");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    require_df(c);
+    FH_CHECK((T1 == nullptr) == (T2 == nullptr), -1, "T1 and T2 must both be given or both NULL");
+    if (T1) {
+      Buf a = dalloc(c, c.nov()), b = dalloc(c, c.n2());
+      h2d(c, a.d(), T1, c.nov());
+      h2d(c, b.d(), T2, c.n2());
+      *E = energy(c, a.d(), b.d());
+    } else {
+      FH_CHECK(c.have_amp, -6, "no device-resident amplitudes");
+      *E = energy(c, c.T1.d(), c.T2.d());
+    }
+    flush_timers(c);
+    FH_CHECK(std::isfinite(*E), -5, "non-finite energy");
+  });
+}
+
+int fh_ccsd_residual(fh_ctx* x, const double* T1, const double* T2, double* n1, double* n2) {
+  return guarded([&] {
+    FH_CHECK(x && T1 && T2 && n1 && n2, -1, "null argument");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    require_df(c);
+    Buf a = dalloc(c, c.nov()), b = dalloc(c, c.n2()), ra = dalloc(c, c.nov()), rb = dalloc(c, c.n2());
+    h2d(c, a.d(), T1, c.nov());
+    h2d(c, b.d(), T2, c.n2());
+    double r1, r2;
+    update_amp(c, a.d(), b.d(), ra.d(), rb.d(), nullptr, nullptr, false, &r1, &r2);
+    d2h(c, n1, ra.d(), c.nov());
+    d2h(c, n2, rb.d(), c.n2());
+    flush_timers(c);
+  });
+}
+
+int fh_ccsd_solve(fh_ctx* x, const fh_cc_opts* opts, double* T1, double* T2, fh_cc_result* res) {
+  return guarded([&] {
+    FH_CHECK(x && res, -1, "null argument");
+    FH_CHECK((T1 == nullptr) == (T2 == nullptr), -1, "T1 and T2 must both be given or both NULL");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    require_df(c);
+    fh_cc_opts op;
+    fh_default_opts(&op);
+    if (opts) op = *opts;
+    if (T1) upload_amp(c, T1, T2);
+    else if (!c.have_amp) mp2_guess(c);
+    std::memset(res, 0, sizeof(*res));
+    solve(c, op, *res);
+    if (T1) {
+      d2h(c, T1, c.T1.d(), c.nov());
+      d2h(c, T2, c.T2.d(), c.n2());
+    }
+    FH_HIP(hipStreamSynchronize(c.s));
+    flush_timers(c);
+  });
+}
+
+int fh_triples(fh_ctx* x, int prec_mode, const double* T1, const double* T2, double* Et) {
+  return guarded([&] {
+    FH_CHECK(x && Et, -1, "null argument");
+    FH_CHECK(prec_mode == 0, -1, "fh_triples: only prec_mode 0 (FP64) is available");
+    FH_CHECK((T1 == nullptr) == (T2 == nullptr), -1, "T1 and T2 must both be given or both NULL");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    require_df(c);
+    if (T1) upload_amp(c, T1, T2);
+    *Et = triples(c);
+    flush_timers(c);
+    FH_CHECK(std::isfinite(*Et), -5, "non-finite (T) energy");
+  });
+}
+
+int fh_stats(fh_ctx* x, char* json, size_t cap) {
+  return guarded([&] {
+    FH_CHECK(x && json && cap > 0, -1, "null argument");
+    Ctx& c = x->c;
+    DeviceGuard g(c.device);
+    FH_HIP(hipStreamSynchronize(c.s));
+    flush_timers(c);
+    std::ostringstream s;
+    s.precision(17);
+    s << "{\"o\":" << c.o << ",\"v\":" << c.v << ",\"naux\":" << c.naux << ",\"pool_peak_bytes\":" << c.pool->peak()
+      << ",\"phases\":{";
+    bool first = true;
+    for (auto& kv : c.phases) {
+      if (!first) s << ",";
+      first = false;
+      s << "\"" << kv.first << "\":{\"ms\":" << kv.second.ms << ",\"flops\":" << kv.second.flops
+        << ",\"calls\":" << kv.second.calls << "}";
+    }
+    s << "},\"iterations\":[";
+    for (size_t q = 0; q < c.iters.size(); ++q) {
+      const IterRow& r = c.iters[q];
+      if (q) s << ",";
+      s << "[" << r.ite << "," << r.e << "," << r.de << "," << r.rms << "," << r.ms << "," << (r.diis ? "true" : "false") << "]";
+    }
+    s << "]}";
+    const std::string out = s.str();
+    FH_CHECK(out.size() + 1 <= cap, -1, "fh_stats: buffer too small (" + std::to_string(out.size() + 1) + " bytes needed)");
+    std::memcpy(json, out.c_str(), out.size() + 1);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+// FP64 GEMM on CDNA4 matrix cores: D = alpha * op(A) * op(B) + beta * C, column-major.
+//
+// Workgroup: 256 threads = 4 wavefronts in a 2x2 arrangement; each wavefront owns a
+// (BM/2)x(BN/2) block of the output as 16x16 v_mfma_f64_16x16x4f64 accumulators.
+// Operand tiles are staged through LDS with double buffering (global loads for stage
+// s+1 are in flight while stage s is multiplied).  LDS row paddings are chosen so that
+// the 64-bit fragment reads of a half-wavefront hit 32 distinct bank pairs.
+//
+// Tile -> workgroup mapping is XCD aware: consecutive workgroup ids land on different
+// XCDs (round robin over 8), so the linear id is remapped such that each XCD receives a
+// contiguous range of output tiles, M-tile fastest.  Tiles that share a column block of B
+// then run on the same XCD and hit its L2.
+#include "fh_internal.h"
+
+namespace fh {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <int BM, int BN, int BK, bool TA, bool TB, int OCC, int TAG>
+__global__ __launch_bounds__(256, OCC) void k_gemm_f64(
+    int64_t M, int64_t N, int64_t K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
+    const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
+    int64_t sC, double* D, int64_t ldd, int64_t sD, int tilesM, int tilesN) {
+  constexpr int PADA = 16, PADB = 2;
+  constexpr int WM = BM / 2, WN = BN / 2, MB = WM / 16, NB = WN / 16;
+  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
+  __shared__ double As[2][BK][BM + PADA];
+  __shared__ double Bs[2][BN][BK + PADB];
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid & 1, wn = wid >> 1;
+
+  // XCD-aware remap of the linear tile id
+  const int64_t nwg = (int64_t)tilesM * tilesN;
+  const int64_t w = blockIdx.x;
+  const int64_t q = nwg >> 3, r = nwg & 7, x = w & 7;
+  const int64_t L = x * q + (x < r ? x : r) + (w >> 3);
+  const int64_t mt = L % tilesM, nt = L / tilesM;
+  const int64_t m0 = mt * BM, n0 = nt * BN;
+
+  const int64_t bz = blockIdx.y;
+  A += bz * sA;
+  B += bz * sB;
+  if (C) C += bz * sC;
+  D += bz * sD;
+
+  double ra[LA], rb[LB];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int t = 0; t < LA; ++t) {
+      const int e = tid + 256 * t;
+      int mm, kk;
+      if (!TA) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
+      const int64_t gm = m0 + mm, gk = k0 + kk;
+      double val = 0.0;
+      if (gm < M && gk < K) val = TA ? A[gk + lda * gm] : A[gm + lda * gk];
+      ra[t] = val;
+    }
+#pragma unroll
+    for (int t = 0; t < LB; ++t) {
+      const int e = tid + 256 * t;
+      int nn, kk;
+      if (!TB) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
+      const int64_t gn = n0 + nn, gk = k0 + kk;
+      double val = 0.0;
+      if (gn < N && gk < K) val = TB ? B[gn + ldb * gk] : B[gk + ldb * gn];
+      rb[t] = val;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < LA; ++t) {
+      const int e = tid + 256 * t;
+      int mm, kk;
+      if (!TA) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
+      As[buf][kk][mm] = ra[t];
+    }
+#pragma unroll
+    for (int t = 0; t < LB; ++t) {
+      const int e = tid + 256 * t;
+      int nn, kk;
+      if (!TB) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
+      Bs[buf][nn][kk] = rb[t];
+    }
+  };
+
+  f64x4 acc[MB][NB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  const int64_t nst = (K + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int64_t s = 0; s < nst; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nst) gload((s + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      double af[MB], bf[NB];
+#pragma unroll
+      for (int i = 0; i < MB; ++i) af[i] = As[buf][kk * 4 + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) bf[j] = Bs[buf][wn * WN + j * 16 + (lane & 15)][kk * 4 + (lane >> 4)];
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    if (s + 1 < nst) sstore(buf ^ 1);
+    __syncthreads();
+  }
+
+  // epilogue: lane holds rows (lane>>4)+4r, column lane&15 of each 16x16 block
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int64_t col = n0 + wn * WN + j * 16 + (lane & 15);
+      if (col >= N) continue;
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int64_t row = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * rr;
+        if (row < M) {
+          double val = alpha * acc[i][j][rr];
+          if (beta != 0.0) val += beta * C[row + ldc * col];
+          D[row + ldd * col] = val;
+        }
+      }
+    }
+}
+
+template <int BM, int BN, bool TA, bool TB, int TAG>
+static void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                   int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
+                   int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch) {
+  const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+  FH_CHECK(tm * tn < (int64_t)1 << 31 && batch < 65536, -3, "gemm grid too large");
+  dim3 grid((unsigned)(tm * tn), (unsigned)batch);
+  hipLaunchKernelGGL((k_gemm_f64<BM, BN, 16, TA, TB, (BM * BN > 8192 ? 1 : 2), TAG>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB,
+                     beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn);
+  FH_HIP(hipGetLastError());
+}
+
+template <bool TA, bool TB, int TAG>
+static void pick(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                 int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
+                 int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch) {
+  const int64_t big = ((M + 127) / 128) * ((N + 127) / 128) * batch;
+  if (big >= 512)
+    launch<128, 128, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch);
+  else
+    launch<64, 64, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch);
+}
+
+void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+              int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C,
+              int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int tag) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  FH_CHECK(K >= 0, -3, "gemm: negative K");
+  FH_CHECK(lda >= (ta ? (K > 0 ? K : 1) : M) && ldb >= (tb ? N : (K > 0 ? K : 1)) && ldd >= M, -3,
+           "gemm: leading dimension too small");
+  FH_CHECK(beta == 0.0 || (C != nullptr && ldc >= M), -3, "gemm: C missing");
+#define FH_PICK(TA_, TB_)                                                                                   \
+  do {                                                                                                     \
+    if (tag == 1) pick<TA_, TB_, 1>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch); \
+    else if (tag == 2) pick<TA_, TB_, 2>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch); \
+    else pick<TA_, TB_, 0>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch); \
+  } while (0)
+  if (ta) {
+    if (tb) FH_PICK(true, true); else FH_PICK(true, false);
+  } else {
+    if (tb) FH_PICK(false, true); else FH_PICK(false, false);
+  }
+#undef FH_PICK
+}
+
+}  // namespace fh

@@ -1,0 +1,127 @@
+// Internal declarations shared by the libfermihip translation units.
+// All tensors are column-major (first index fastest), like the Julia arrays that
+// cross the C-ABI.  Everything here runs on one HIP stream owned by the context.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include <map>
+
+namespace fh {
+
+// ---------------------------------------------------------------- errors
+struct Error {
+  int code;
+  std::string msg;
+};
+void set_error(const std::string& m);
+#define FH_HIP(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      throw ::fh::Error{-2, std::string("HIP error ") + hipGetErrorString(e_) + " at " + __FILE__ + ":" + std::to_string(__LINE__)}; \
+  } while (0)
+#define FH_CHECK(cond, code, msg)                  \
+  do {                                             \
+    if (!(cond)) throw ::fh::Error{(code), (msg)}; \
+  } while (0)
+
+// ---------------------------------------------------------------- device memory pool
+// Size-bucketed caching allocator: buffers are returned to a free list and reused;
+// nothing is released to HIP until the pool is destroyed.
+class Pool {
+ public:
+  explicit Pool(int device) : device_(device) {}
+  ~Pool();
+  void* get(size_t bytes);
+  void put(void* p);
+  size_t in_use() const { return in_use_; }
+  size_t peak() const { return peak_; }
+  void release_free();
+ private:
+  int device_;
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> live_;
+  size_t in_use_ = 0, peak_ = 0;
+};
+
+// Owning device buffer handed out by the pool.
+struct Buf {
+  Pool* pool = nullptr;
+  void* p = nullptr;
+  size_t bytes = 0;
+  Buf() = default;
+  Buf(Pool* pl, size_t b) : pool(pl), p(b ? pl->get(b) : nullptr), bytes(b) {}
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  Buf(Buf&& o) noexcept { *this = std::move(o); }
+  Buf& operator=(Buf&& o) noexcept {
+    if (this != &o) { reset(); pool = o.pool; p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~Buf() { reset(); }
+  void reset() { if (p && pool) pool->put(p); p = nullptr; bytes = 0; }
+  double* d() const { return static_cast<double*>(p); }
+  float* f() const { return static_cast<float*>(p); }
+};
+
+// A column-major FP64 tensor view (up to 6 modes).
+struct TView {
+  double* p = nullptr;
+  int nd = 0;
+  int64_t dim[6] = {1, 1, 1, 1, 1, 1};
+  int64_t size() const { int64_t s = 1; for (int i = 0; i < nd; ++i) s *= dim[i]; return s; }
+};
+inline TView tv(double* p, std::initializer_list<int64_t> dims) {
+  TView t; t.p = p; t.nd = 0;
+  for (int64_t d : dims) t.dim[t.nd++] = d;
+  return t;
+}
+
+// ---------------------------------------------------------------- kernel launchers
+// D = alpha * op(A) * op(B) + beta * C  (column-major, FP64 MFMA); C may alias D.
+// Strided batches: A_i = A + i*sA etc.
+void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha,
+              const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB,
+              double beta, const double* C, int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD,
+              int64_t batch = 1, int tag = 0);
+// tag selects a distinct kernel symbol for profiling: 0 generic, 1 PP ladder, 2 (T) W build
+
+// out[perm] = alpha * in + beta * out: out dim d has size odim[d] and reads input with stride istr[d].
+void permute_f64(hipStream_t s, int nd, const int64_t* odim, const int64_t* istr, double alpha,
+                 const double* in, double beta, double* out);
+
+void axpby_f64(hipStream_t s, int64_t n, double a, const double* x, double b, double* y);  // y = a x + b y
+void fill_f64(hipStream_t s, int64_t n, double v, double* y);
+// out[i,j,a,b] = t2 + t1[i,a] t1[j,b]
+void tau_f64(hipStream_t s, int o, int v, const double* t1, const double* t2, double c, double* out);
+// T1/T2 Jacobi update: new = R / D (D from eps on the fly); writes new into R, err = new - old
+// (as FP64 or FP32) and per-block partial sums of err^2.
+void jacobi_update(hipStream_t s, int o, int v, bool is_t2, const double* eo, const double* ev, double* R,
+                   const double* old, void* err, bool err_f32, double* partials, int64_t* nblocks);
+int64_t reduce_blocks_needed(int64_t n);
+// deterministic FP64 dot products: partial sums per block, then one final block.
+void dot_f64(hipStream_t s, int64_t n, const double* x, const double* y, double* partials, double* result);
+void dot_f32(hipStream_t s, int64_t n, const float* x, const float* y, double* partials, double* result);
+void sum_partials(hipStream_t s, int64_t n, const double* partials, double* result);
+// y = sum_i c_i x_i (+ sum_i c_i e_i) for up to 8 FP64 vectors and 8 FP64/FP32 error vectors
+void lincomb(hipStream_t s, int64_t n, int nv, const double* const* xs, const double* cs,
+             const void* const* es, bool err_f32, bool add_err, double* y);
+void cast_f64_to_f32(hipStream_t s, int64_t n, const double* x, float* y);
+// closed-shell CC energy (RCCSDHelper.jl:8-23) -> partials + final sum into result
+void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, const double* ovov,
+               double* partials, double* result);
+
+// (T) energy for one (i,j) pair and k in [k0, k0+nk): sums into partials[blockIdx...].
+struct TriplesDesc {
+  // W[a,b,c] for occupied k = sum over the six classes of G[c][a*sa + b*sb + c*sc + k*sk]
+  const double* G[6];
+  int64_t sa[6], sb[6], sc[6], sk[6];
+};
+void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TriplesDesc& d,
+                    const double* t1p, const double* vvoo, const double* eo, const double* ev,
+                    double* partials, int64_t* nblocks);
+
+}  // namespace fh

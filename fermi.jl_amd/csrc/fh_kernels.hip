@@ -1,0 +1,393 @@
+// Bandwidth-bound kernels of the DF-RCCSD(T) engine: tensor permutes, amplitude
+// updates with on-the-fly denominators, deterministic FP64 reductions, DIIS linear
+// combinations, the CC energy and the (T) energy epilogue.
+//
+// Reductions are deterministic: every kernel writes one FP64 partial per workgroup
+// (fixed grid for a given length) and a single workgroup adds the partials in a fixed
+// order, so repeated runs give bitwise-identical results.
+#include "fh_internal.h"
+
+namespace fh {
+
+static constexpr int RB = 256;        // threads per reduction block
+static constexpr int64_t RMAX = 2048; // max partials per reduction
+
+int64_t reduce_blocks_needed(int64_t n) {
+  int64_t b = (n + RB * 4 - 1) / (RB * 4);
+  if (b < 1) b = 1;
+  if (b > RMAX) b = RMAX;
+  return b;
+}
+
+static __device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// block-wide sum, result valid in thread 0
+static __device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[RB / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < RB / 64; ++i) t += red[i];
+  }
+  return t;
+}
+
+// ------------------------------------------------------------------ permute
+struct PermArgs {
+  int nd;
+  int64_t odim[6], istr[6], ostr[6];
+};
+
+// out element per thread; used when output dim 0 reads the input with unit stride
+__global__ __launch_bounds__(256) void k_permute_simple(PermArgs a, int64_t n, double alpha, const double* __restrict__ in,
+                                                        double beta, double* __restrict__ out) {
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * 256) {
+    int64_t rem = idx, off = 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      if (d < a.nd) {
+        const int64_t c = rem % a.odim[d];
+        rem /= a.odim[d];
+        off += c * a.istr[d];
+      }
+    }
+    double v = alpha * in[off];
+    if (beta != 0.0) v += beta * out[idx];
+    out[idx] = v;
+  }
+}
+
+// 32x32 tiles over (output dim 0, output dim p) where dim p is the input's unit-stride dim:
+// reads are coalesced along p, writes along output dim 0.
+__global__ __launch_bounds__(256) void k_permute_tiled(PermArgs a, int p, int64_t nrest, int64_t t0, int64_t tp,
+                                                       double alpha, const double* __restrict__ in, double beta,
+                                                       double* __restrict__ out) {
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int64_t blk = blockIdx.x; blk < nrest * t0 * tp; blk += gridDim.x) {
+    const int64_t b0 = blk % t0, bp = (blk / t0) % tp, rest = blk / (t0 * tp);
+    // decompose the remaining dims
+    int64_t rem = rest, ioff = 0, ooff = 0;
+    for (int d = 1; d < a.nd; ++d) {
+      if (d == p) continue;
+      const int64_t c = rem % a.odim[d];
+      rem /= a.odim[d];
+      ioff += c * a.istr[d];
+      ooff += c * a.ostr[d];
+    }
+    // load: threads consecutive along p
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t i0 = b0 * 32 + ty + 8 * r, ip = bp * 32 + tx;
+      if (i0 < a.odim[0] && ip < a.odim[p]) tile[ty + 8 * r][tx] = in[ioff + i0 * a.istr[0] + ip];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t i0 = b0 * 32 + tx, ip = bp * 32 + ty + 8 * r;
+      if (i0 < a.odim[0] && ip < a.odim[p]) {
+        const int64_t o = ooff + i0 + ip * a.ostr[p];
+        double v = alpha * tile[tx][ty + 8 * r];
+        if (beta != 0.0) v += beta * out[o];
+        out[o] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void permute_f64(hipStream_t s, int nd, const int64_t* odim, const int64_t* istr, double alpha, const double* in,
+                 double beta, double* out) {
+  FH_CHECK(nd >= 1 && nd <= 6, -3, "permute: rank out of range");
+  PermArgs a;
+  a.nd = nd;
+  int64_t n = 1;
+  for (int d = 0; d < 6; ++d) { a.odim[d] = 1; a.istr[d] = 0; a.ostr[d] = 0; }
+  for (int d = 0; d < nd; ++d) {
+    a.odim[d] = odim[d];
+    a.istr[d] = istr[d];
+    a.ostr[d] = n;
+    n *= odim[d];
+  }
+  if (n == 0) return;
+  int p = -1;
+  if (a.istr[0] != 1 || a.odim[0] == 1) {
+    for (int d = 1; d < nd; ++d)
+      if (a.istr[d] == 1 && a.odim[d] > 1) p = d;
+  }
+  if (p < 0 || a.odim[0] < 8) {
+    int64_t g = (n + 255) / 256;
+    if (g > 65536 * 4) g = 65536 * 4;
+    hipLaunchKernelGGL(k_permute_simple, dim3((unsigned)g), dim3(256), 0, s, a, n, alpha, in, beta, out);
+  } else {
+    const int64_t t0 = (a.odim[0] + 31) / 32, tp = (a.odim[p] + 31) / 32;
+    const int64_t nrest = n / (a.odim[0] * a.odim[p]);
+    int64_t g = nrest * t0 * tp;
+    if (g > 65536 * 4) g = 65536 * 4;
+    hipLaunchKernelGGL(k_permute_tiled, dim3((unsigned)g), dim3(256), 0, s, a, p, nrest, t0, tp, alpha, in, beta, out);
+  }
+  FH_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ elementwise
+__global__ void k_axpby(int64_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    y[i] = (b == 0.0) ? a * x[i] : a * x[i] + b * y[i];
+}
+__global__ void k_fill(int64_t n, double v, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) y[i] = v;
+}
+static unsigned grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+void axpby_f64(hipStream_t s, int64_t n, double a, const double* x, double b, double* y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(256), 0, s, n, a, x, b, y);
+  FH_HIP(hipGetLastError());
+}
+void fill_f64(hipStream_t s, int64_t n, double v, double* y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, n, v, y);
+  FH_HIP(hipGetLastError());
+}
+
+// tau[i,j,a,b] = t2[i,j,a,b] + c * t1[i,a] t1[j,b]
+__global__ void k_tau(int o, int v, const double* __restrict__ t1, const double* __restrict__ t2, double c,
+                      double* __restrict__ out) {
+  const int64_t n = (int64_t)o * o * v * v;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * 256) {
+    const int64_t i = idx % o, j = (idx / o) % o, a = (idx / ((int64_t)o * o)) % v, b = idx / ((int64_t)o * o * v);
+    out[idx] = t2[idx] + c * t1[i + o * a] * t1[j + o * b];
+  }
+}
+void tau_f64(hipStream_t s, int o, int v, const double* t1, const double* t2, double c, double* out) {
+  const int64_t n = (int64_t)o * o * v * v;
+  hipLaunchKernelGGL(k_tau, dim3(grid_for(n)), dim3(256), 0, s, o, v, t1, t2, c, out);
+  FH_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ Jacobi update (RCCSDHelper.jl:347-348)
+template <bool T2, bool F32>
+__global__ __launch_bounds__(RB) void k_jacobi(int o, int v, const double* __restrict__ eo, const double* __restrict__ ev,
+                                               double* __restrict__ R, const double* __restrict__ old, void* err,
+                                               double* __restrict__ partials) {
+  const int64_t n = T2 ? (int64_t)o * o * v * v : (int64_t)o * v;
+  double acc = 0.0;
+  for (int64_t idx = (int64_t)blockIdx.x * RB + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * RB) {
+    double d;
+    if (T2) {
+      const int64_t i = idx % o, j = (idx / o) % o, a = (idx / ((int64_t)o * o)) % v, b = idx / ((int64_t)o * o * v);
+      d = eo[i] + eo[j] - ev[a] - ev[b];
+    } else {
+      const int64_t i = idx % o, a = idx / o;
+      d = eo[i] - ev[a];
+    }
+    const double nv = R[idx] / d;
+    R[idx] = nv;
+    const double e = nv - old[idx];
+    if (err) {
+      if (F32) static_cast<float*>(err)[idx] = (float)e;
+      else static_cast<double*>(err)[idx] = e;
+    }
+    acc += e * e;
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+void jacobi_update(hipStream_t s, int o, int v, bool is_t2, const double* eo, const double* ev, double* R,
+                   const double* old, void* err, bool err_f32, double* partials, int64_t* nblocks) {
+  const int64_t n = is_t2 ? (int64_t)o * o * v * v : (int64_t)o * v;
+  const int64_t g = reduce_blocks_needed(n);
+  *nblocks = g;
+  if (is_t2) {
+    if (err_f32) hipLaunchKernelGGL((k_jacobi<true, true>), dim3(g), dim3(RB), 0, s, o, v, eo, ev, R, old, err, partials);
+    else hipLaunchKernelGGL((k_jacobi<true, false>), dim3(g), dim3(RB), 0, s, o, v, eo, ev, R, old, err, partials);
+  } else {
+    if (err_f32) hipLaunchKernelGGL((k_jacobi<false, true>), dim3(g), dim3(RB), 0, s, o, v, eo, ev, R, old, err, partials);
+    else hipLaunchKernelGGL((k_jacobi<false, false>), dim3(g), dim3(RB), 0, s, o, v, eo, ev, R, old, err, partials);
+  }
+  FH_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ reductions
+template <typename T>
+__global__ __launch_bounds__(RB) void k_dot(int64_t n, const T* __restrict__ x, const T* __restrict__ y,
+                                            double* __restrict__ partials) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
+    acc += (double)x[i] * (double)y[i];
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+__global__ __launch_bounds__(RB) void k_sum(int64_t n, const double* __restrict__ partials, double* __restrict__ result) {
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += RB) acc += partials[i];
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) *result = t;
+}
+void sum_partials(hipStream_t s, int64_t n, const double* partials, double* result) {
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, s, n, partials, result);
+  FH_HIP(hipGetLastError());
+}
+void dot_f64(hipStream_t s, int64_t n, const double* x, const double* y, double* partials, double* result) {
+  const int64_t g = reduce_blocks_needed(n);
+  hipLaunchKernelGGL((k_dot<double>), dim3(g), dim3(RB), 0, s, n, x, y, partials);
+  FH_HIP(hipGetLastError());
+  sum_partials(s, g, partials, result);
+}
+void dot_f32(hipStream_t s, int64_t n, const float* x, const float* y, double* partials, double* result) {
+  const int64_t g = reduce_blocks_needed(n);
+  hipLaunchKernelGGL((k_dot<float>), dim3(g), dim3(RB), 0, s, n, x, y, partials);
+  FH_HIP(hipGetLastError());
+  sum_partials(s, g, partials, result);
+}
+
+// ------------------------------------------------------------------ DIIS extrapolation (DIIS.jl:86-93)
+struct LinArgs {
+  int nv;
+  const double* x[8];
+  const void* e[8];
+  double c[8];
+};
+template <bool F32>
+__global__ void k_lincomb(int64_t n, LinArgs a, bool add_err, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double acc = 0.0;
+    for (int q = 0; q < a.nv; ++q) acc += a.c[q] * a.x[q][i];
+    if (add_err) {
+      double ae = 0.0;
+      for (int q = 0; q < a.nv; ++q)
+        ae += a.c[q] * (F32 ? (double)static_cast<const float*>(a.e[q])[i] : static_cast<const double*>(a.e[q])[i]);
+      acc += ae;
+    }
+    y[i] = acc;
+  }
+}
+void lincomb(hipStream_t s, int64_t n, int nv, const double* const* xs, const double* cs, const void* const* es,
+             bool err_f32, bool add_err, double* y) {
+  FH_CHECK(nv >= 1 && nv <= 8, -3, "lincomb: 1..8 vectors");
+  LinArgs a;
+  a.nv = nv;
+  for (int q = 0; q < 8; ++q) { a.x[q] = q < nv ? xs[q] : nullptr; a.e[q] = q < nv ? es[q] : nullptr; a.c[q] = q < nv ? cs[q] : 0.0; }
+  if (err_f32) hipLaunchKernelGGL((k_lincomb<true>), dim3(grid_for(n)), dim3(256), 0, s, n, a, add_err, y);
+  else hipLaunchKernelGGL((k_lincomb<false>), dim3(grid_for(n)), dim3(256), 0, s, n, a, add_err, y);
+  FH_HIP(hipGetLastError());
+}
+
+__global__ void k_cast(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) y[i] = (float)x[i];
+}
+void cast_f64_to_f32(hipStream_t s, int64_t n, const double* x, float* y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cast, dim3(grid_for(n)), dim3(256), 0, s, n, x, y);
+  FH_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ CC energy (RCCSDHelper.jl:8-23)
+// E = sum_{k,c,l,d} (kc|ld) * (2 T2[k,l,c,d] - T2[l,k,c,d] - T1[l,c] T1[k,d] + 2 T1[k,c] T1[l,d]),
+// iterated in the memory order of OVOV[k,c,l,d] so that the integral stream is coalesced.
+__global__ __launch_bounds__(RB) void k_cc_energy(int o, int v, const double* __restrict__ t1, const double* __restrict__ t2,
+                                                  const double* __restrict__ ovov, double* __restrict__ partials) {
+  const int64_t n = (int64_t)o * v * o * v;
+  const int64_t oo = (int64_t)o * o;
+  double acc = 0.0;
+  for (int64_t idx = (int64_t)blockIdx.x * RB + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * RB) {
+    const int64_t k = idx % o, c = (idx / o) % v, l = (idx / ((int64_t)o * v)) % o, d = idx / ((int64_t)o * v * o);
+    const double t2klcd = t2[k + o * l + oo * c + oo * v * d];
+    const double t2lkcd = t2[l + o * k + oo * c + oo * v * d];
+    const double b = 2.0 * t2klcd - t2lkcd - t1[l + o * c] * t1[k + o * d] + 2.0 * t1[k + o * c] * t1[l + o * d];
+    acc += b * ovov[idx];
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, const double* ovov, double* partials,
+               double* result) {
+  const int64_t n = (int64_t)o * v * o * v;
+  const int64_t g = reduce_blocks_needed(n);
+  hipLaunchKernelGGL(k_cc_energy, dim3(g), dim3(RB), 0, s, o, v, t1, t2, ovov, partials);
+  FH_HIP(hipGetLastError());
+  sum_partials(s, g, partials, result);
+}
+
+// ------------------------------------------------------------------ (T) energy epilogue (ijk.jl:116-136)
+// One thread per virtual pair a>=b (decoded from the linear pair index), loop over c<=b.
+// W at the six permutations of (a,b,c) is gathered from the six GEMM class buffers; V adds
+// the T1 x (ia|jb) terms; the energy uses the reference's restricted-sum weights.
+__device__ __forceinline__ double tW(const TriplesDesc& d, int64_t a, int64_t b, int64_t c, int64_t k) {
+  double w = 0.0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) w += d.G[q][a * d.sa[q] + b * d.sb[q] + c * d.sc[q] + k * d.sk[q]];
+  return w;
+}
+
+__global__ __launch_bounds__(RB) void k_triples_energy(int o, int v, int i, int j, int k0, TriplesDesc d,
+                                                       const double* __restrict__ t1p, const double* __restrict__ vvoo,
+                                                       const double* __restrict__ eo, const double* __restrict__ ev,
+                                                       double* __restrict__ partials) {
+  const int64_t npair = (int64_t)v * (v + 1) / 2;
+  const int kk = blockIdx.y;
+  const int k = k0 + kk;
+  const double dij = (i == j) ? 1.0 : 0.0, djk = (j == k) ? 1.0 : 0.0;
+  const double wocc = 2.0 - dij - djk;
+  const double Dijk = eo[i] + eo[j] + eo[k];
+  const int64_t vv = (int64_t)v * v;
+  const double* vjk = vvoo + vv * (j + (int64_t)o * k);  // vvoo[:,:,j,k]
+  const double* vik = vvoo + vv * (i + (int64_t)o * k);
+  const double* vij = vvoo + vv * (i + (int64_t)o * j);
+  double acc = 0.0;
+  const int64_t p = (int64_t)blockIdx.x * RB + threadIdx.x;
+  if (wocc != 0.0 && p < npair) {
+    int64_t a = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while (a * (a + 1) / 2 > p) --a;
+    while ((a + 1) * (a + 2) / 2 <= p) ++a;
+    const int64_t b = p - a * (a + 1) / 2;
+    const double ta = t1p[a + (int64_t)v * i], tb = t1p[b + (int64_t)v * j];
+    const double Dab = Dijk - ev[a] - ev[b];
+    const double dab = (a == b) ? 1.0 : 0.0;
+    for (int64_t c = 0; c <= b; ++c) {
+      const double Wabc = tW(d, a, b, c, kk), Wacb = tW(d, a, c, b, kk), Wbac = tW(d, b, a, c, kk);
+      const double Wbca = tW(d, b, c, a, kk), Wcab = tW(d, c, a, b, kk), Wcba = tW(d, c, b, a, kk);
+      // V[x,y,z] = W[x,y,z] + T1p[x,i] vvoo[y,z,j,k] + vvoo[x,z,i,k] T1p[y,j] + vvoo[x,y,i,j] T1p[z,k]
+      auto Vf = [&](int64_t x, int64_t y, int64_t z, double w) {
+        return w + t1p[x + (int64_t)v * i] * vjk[y + v * z] + vik[x + v * z] * t1p[y + (int64_t)v * j] +
+               vij[x + v * y] * t1p[z + (int64_t)v * k];
+      };
+      const double Vabc = Vf(a, b, c, Wabc), Vacb = Vf(a, c, b, Wacb), Vbac = Vf(b, a, c, Wbac);
+      const double Vbca = Vf(b, c, a, Wbca), Vcab = Vf(c, a, b, Wcab), Vcba = Vf(c, b, a, Wcba);
+      const double X = Wabc * Vabc + Wacb * Vacb + Wbac * Vbac + Wbca * Vbca + Wcab * Vcab + Wcba * Vcba;
+      const double Y = Vabc + Vbca + Vcab;
+      const double Z = Vacb + Vbac + Vcba;
+      const double Ef = (Y - 2.0 * Z) * (Wabc + Wbca + Wcab) + (Z - 2.0 * Y) * (Wacb + Wbac + Wcba) + 3.0 * X;
+      const double Dd = Dab - ev[c];
+      const double dbc = (b == c) ? 1.0 : 0.0;
+      acc += Ef * wocc / (Dd * (1.0 + dab + dbc));
+    }
+    (void)ta; (void)tb;
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x + (int64_t)gridDim.x * kk] = t;
+}
+
+void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TriplesDesc& d, const double* t1p,
+                    const double* vvoo, const double* eo, const double* ev, double* partials, int64_t* nblocks) {
+  const int64_t npair = (int64_t)v * (v + 1) / 2;
+  const int64_t gx = (npair + RB - 1) / RB;
+  *nblocks = gx * nk;
+  hipLaunchKernelGGL(k_triples_energy, dim3((unsigned)gx, (unsigned)nk), dim3(RB), 0, s, o, v, i, j, k0, d, t1p, vvoo,
+                     eo, ev, partials);
+  FH_HIP(hipGetLastError());
+}
+
+}  // namespace fh

@@ -1,0 +1,308 @@
+"""Python host side of libfermihip (ctypes), mirroring Fermi.jl's RCCSD / RCCSD(T) interface.
+
+Fermi.jl selects a CC backend by algorithm singleton types (RCCSD.jl:12,76-88;
+PerturbativeTriples.jl:1-11).  This module offers the same call shapes on top of the
+C ABI in ``include/fermihip.h``:
+
+  * ``RCCSD(moints, alg=RCCSDhip())``               -> MP2 guess + iterations  (RCCSDa.jl:46-90)
+  * ``RCCSD(moints, T1, T2, alg=RCCSDhip())``       -> iterations from given amplitudes (RCCSDa.jl:92-245)
+  * ``RCCSDpT(ccsd, moints, alg=ijkhip())``         -> (T) correction (ijk.jl:20-150)
+
+``moints`` is a :class:`MOIntegrals`, the subset of Fermi's ``IntegralHelper`` cache the path
+reads: ``BOO[Q,o,o]``, ``BOV[Q,o,v]``, ``BVV[Q,v,v]``, ``Fii[o]``, ``Faa[v]`` and the
+reference energy.  Options use Fermi's key names (Options.jl:62-102).  Errors raise
+:class:`FermiException` with the library's message; non-convergence is not an error.
+
+There is no CPU fallback: if ``libfermihip.so`` is missing or no HIP device is present,
+every entry point raises.
+"""
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+__all__ = [
+    "FermiException", "MOIntegrals", "Options", "RCCSDhip", "ijkhip", "RCCSD", "RCCSDpT", "Context",
+    "lib_path", "load_library", "EXPORTED_SYMBOLS",
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EXPORTED_SYMBOLS = [
+    "fh_default_opts", "fh_create", "fh_set_df", "fh_prepare", "fh_t2_guess", "fh_ccsd_energy",
+    "fh_ccsd_residual", "fh_ccsd_solve", "fh_triples", "fh_stats", "fh_last_error", "fh_destroy",
+]
+
+
+class FermiException(Exception):
+    """Mirror of Fermi.Options.FermiException (Options.jl:196-199)."""
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [
+        ("cc_max_iter", ctypes.c_int), ("cc_e_conv", ctypes.c_double), ("cc_max_rms", ctypes.c_double),
+        ("cc_diis", ctypes.c_int), ("cc_ndiis", ctypes.c_int), ("diis_start", ctypes.c_int),
+        ("cc_diis_relax", ctypes.c_int), ("diis_prec", ctypes.c_int), ("cc_damp_ratio", ctypes.c_double),
+        ("precision_override", ctypes.c_int), ("verbose", ctypes.c_int),
+    ]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("guess_energy", ctypes.c_double), ("correlation", ctypes.c_double), ("e_conv", ctypes.c_double),
+        ("t_conv", ctypes.c_double), ("niter", ctypes.c_int), ("converged", ctypes.c_int),
+    ]
+
+
+def lib_path():
+    return os.environ.get("FERMIHIP_LIB", os.path.join(HERE, "libfermihip.so"))
+
+
+_LIB = None
+
+
+def load_library():
+    """Load libfermihip.so (built in-tree by build.py).  Raises if it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    p = lib_path()
+    if not os.path.exists(p):
+        raise FermiException(f"libfermihip.so not found at {p}: run __graft_entry__.build()")
+    L = ctypes.CDLL(p)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.fh_default_opts.argtypes = [ctypes.POINTER(_Opts)]
+    L.fh_default_opts.restype = None
+    L.fh_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
+    L.fh_set_df.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp]
+    L.fh_prepare.argtypes = [ctypes.c_void_p]
+    L.fh_t2_guess.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
+    L.fh_ccsd_energy.argtypes = [ctypes.c_void_p, dp, dp, dp]
+    L.fh_ccsd_residual.argtypes = [ctypes.c_void_p, dp, dp, dp, dp]
+    L.fh_ccsd_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Opts), dp, dp, ctypes.POINTER(_Result)]
+    L.fh_triples.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp]
+    L.fh_stats.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+    L.fh_last_error.argtypes = []
+    L.fh_last_error.restype = ctypes.c_char_p
+    L.fh_destroy.argtypes = [ctypes.c_void_p]
+    L.fh_destroy.restype = None
+    for name in EXPORTED_SYMBOLS:
+        if name not in ("fh_default_opts", "fh_last_error", "fh_destroy"):
+            getattr(L, name).restype = ctypes.c_int
+    _LIB = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _colmajor(a):
+    """Julia arrays are column-major: hand the library a Fortran-ordered FP64 copy/view."""
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def Options(**kw):
+    """Fermi option keys read on this path, with the reference defaults (Options.jl:62-102)."""
+    o = dict(cc_max_iter=50, cc_e_conv=1e-10, cc_max_rms=1e-10, cc_diis=True, cc_ndiis=3, diis_start=3,
+             cc_diis_relax=3, diis_prec="single", cc_damp_ratio=0.0, precision_override=False, verbose=False)
+    for k, v in kw.items():
+        if k not in o:
+            raise FermiException(f"unknown option {k}")
+        o[k] = v
+    if o["diis_prec"] not in ("single", "double"):
+        raise FermiException(f"invalid diis_prec {o['diis_prec']!r} (single|double)")
+    return o
+
+
+@dataclass
+class MOIntegrals:
+    """The part of Fermi's IntegralHelper cache read by the CC path (IntegralHelper.jl:50-56)."""
+    BOO: np.ndarray
+    BOV: np.ndarray
+    BVV: np.ndarray
+    Fii: np.ndarray
+    Faa: np.ndarray
+    sd_energy: float = 0.0
+    context: "Context" = field(default=None, repr=False)
+
+    @property
+    def o(self):
+        return self.BOV.shape[1]
+
+    @property
+    def v(self):
+        return self.BOV.shape[2]
+
+    @property
+    def naux(self):
+        return self.BOV.shape[0]
+
+
+class Context:
+    """Owns one fh_ctx (device memory of one molecule)."""
+
+    def __init__(self, device=0):
+        self.L = load_library()
+        h = ctypes.c_void_p()
+        self._check(self.L.fh_create(ctypes.byref(h), int(device)))
+        self.h = h
+        self.o = self.v = self.naux = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FermiException(f"libfermihip error {rc}: {self.L.fh_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_df(self, eps_o, eps_v, Boo, Bov, Bvv):
+        Q, o, v = Bov.shape
+        if Boo.shape != (Q, o, o) or Bvv.shape != (Q, v, v) or len(eps_o) != o or len(eps_v) != v:
+            raise FermiException("inconsistent DF tensor shapes")
+        self._keep = [_colmajor(x) for x in (eps_o, eps_v, Boo, Bov, Bvv)]
+        self._check(self.L.fh_set_df(self.h, o, v, Q, *[_ptr(x) for x in self._keep]))
+        self._keep = None
+        self.o, self.v, self.naux = o, v, Q
+
+    def prepare(self):
+        self._check(self.L.fh_prepare(self.h))
+
+    def guess(self, kind=0, copy_out=True):
+        if not copy_out:
+            self._check(self.L.fh_t2_guess(self.h, kind, None, None))
+            return None, None
+        T1 = np.zeros((self.o, self.v), order="F")
+        T2 = np.zeros((self.o, self.o, self.v, self.v), order="F")
+        self._check(self.L.fh_t2_guess(self.h, kind, _ptr(T1), _ptr(T2)))
+        return T1, T2
+
+    def energy(self, T1=None, T2=None):
+        E = ctypes.c_double()
+        if T1 is None:
+            self._check(self.L.fh_ccsd_energy(self.h, None, None, ctypes.byref(E)))
+        else:
+            a, b = _colmajor(T1), _colmajor(T2)
+            self._check(self.L.fh_ccsd_energy(self.h, _ptr(a), _ptr(b), ctypes.byref(E)))
+        return E.value
+
+    def update_amp(self, T1, T2):
+        a, b = _colmajor(T1), _colmajor(T2)
+        n1 = np.zeros_like(a, order="F")
+        n2 = np.zeros_like(b, order="F")
+        self._check(self.L.fh_ccsd_residual(self.h, _ptr(a), _ptr(b), _ptr(n1), _ptr(n2)))
+        return n1, n2
+
+    def solve(self, opts=None, T1=None, T2=None, copy_out=True):
+        opts = Options() if opts is None else opts
+        c = _Opts()
+        self.L.fh_default_opts(ctypes.byref(c))
+        c.cc_max_iter = int(opts["cc_max_iter"]); c.cc_e_conv = float(opts["cc_e_conv"])
+        c.cc_max_rms = float(opts["cc_max_rms"]); c.cc_diis = int(bool(opts["cc_diis"]))
+        c.cc_ndiis = int(opts["cc_ndiis"]); c.diis_start = int(opts["diis_start"])
+        c.cc_diis_relax = int(opts["cc_diis_relax"]); c.diis_prec = 1 if opts["diis_prec"] == "single" else 0
+        c.cc_damp_ratio = float(opts["cc_damp_ratio"]); c.precision_override = int(bool(opts["precision_override"]))
+        c.verbose = int(bool(opts["verbose"]))
+        r = _Result()
+        if T1 is None:
+            self._check(self.L.fh_ccsd_solve(self.h, ctypes.byref(c), None, None, ctypes.byref(r)))
+            out1 = out2 = None
+            if copy_out:
+                raise FermiException("copy_out requires starting amplitudes; use T1/T2 arrays")
+        else:
+            out1, out2 = _colmajor(T1).copy(order="F"), _colmajor(T2).copy(order="F")
+            self._check(self.L.fh_ccsd_solve(self.h, ctypes.byref(c), _ptr(out1), _ptr(out2), ctypes.byref(r)))
+        return r, out1, out2
+
+    def triples(self, T1=None, T2=None, prec_mode=0):
+        Et = ctypes.c_double()
+        if T1 is None:
+            self._check(self.L.fh_triples(self.h, prec_mode, None, None, ctypes.byref(Et)))
+        else:
+            a, b = _colmajor(T1), _colmajor(T2)
+            self._check(self.L.fh_triples(self.h, prec_mode, _ptr(a), _ptr(b), ctypes.byref(Et)))
+        return Et.value
+
+    def stats(self):
+        import json
+        buf = ctypes.create_string_buffer(1 << 20)
+        self._check(self.L.fh_stats(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+
+# ---------------------------------------------------------------- Fermi-shaped API
+class RCCSDhip:
+    """Algorithm singleton for the MI355X backend (the role of RCCSDa, RCCSD.jl:85-88)."""
+
+
+class ijkhip:
+    """(T) algorithm singleton for the MI355X backend (the role of ijk, PerturbativeTriples.jl:41-43)."""
+
+
+@dataclass
+class RCCSDResult:
+    """Fields of Fermi's RCCSD{T} (RCCSD.jl:61-69)."""
+    guessenergy: float
+    correlation: float
+    energy: float
+    T1: np.ndarray
+    T2: np.ndarray
+    e_conv: float
+    t_conv: float
+    niter: int
+    converged: bool
+
+
+@dataclass
+class RCCSDpTResult:
+    """Fields of Fermi's RCCSDpT{T} (PerturbativeTriples.jl:21-25)."""
+    CCSD: RCCSDResult
+    energy: float
+    correction: float
+
+
+def _ctx_for(moints, device=0):
+    if moints.context is None:
+        moints.context = Context(device)
+        moints.context.set_df(moints.Fii, moints.Faa, moints.BOO, moints.BOV, moints.BVV)
+    return moints.context
+
+
+def RCCSD(moints, *args, alg=None, options=None, device=0):
+    """RCCSD(moints[, T1, T2]; alg) -> RCCSDResult (RCCSDa.jl:46-245 semantics)."""
+    if alg is not None and not isinstance(alg, RCCSDhip):
+        raise FermiException(f"invalid algorithm {type(alg).__name__} for the MI355X RCCSD backend")
+    ctx = _ctx_for(moints, device)
+    opts = Options() if options is None else options
+    if len(args) == 0:
+        T1, T2 = ctx.guess(0)
+    elif len(args) == 2:
+        T1, T2 = args
+        if T1.shape != (moints.o, moints.v) or T2.shape != (moints.o, moints.o, moints.v, moints.v):
+            raise FermiException("amplitude shapes do not match the orbital spaces")
+    else:
+        raise FermiException("invalid arguments for RCCSD method")
+    Eguess = ctx.energy(T1, T2)
+    r, nT1, nT2 = ctx.solve(opts, T1, T2)
+    Eref = float(moints.sd_energy)
+    return RCCSDResult(Eguess + Eref, r.correlation, r.correlation + Eref, nT1, nT2, r.e_conv, r.t_conv, r.niter,
+                       bool(r.converged))
+
+
+def RCCSDpT(ccsd, moints, alg=None, device=0):
+    """RCCSDpT(ccsd, moints; alg) -> RCCSDpTResult (ijk.jl:20-150 semantics)."""
+    if alg is not None and not isinstance(alg, ijkhip):
+        raise FermiException(f"invalid algorithm {type(alg).__name__} for the MI355X (T) backend")
+    ctx = _ctx_for(moints, device)
+    Et = ctx.triples(ccsd.T1, ccsd.T2)
+    return RCCSDpTResult(ccsd, Et + ccsd.energy, Et)

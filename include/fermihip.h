@@ -1,0 +1,106 @@
+/* libfermihip: MI355X-native DF-RCCSD(T) engine, C ABI.
+ *
+ * Drop-in boundary for Fermi.jl's RCCSD / RCCSD(T) algorithm methods.  Julia host code
+ * (or any FFI) passes column-major FP64 arrays exactly as they sit in Fermi's integral
+ * cache (IntegralHelper keys "BOO", "BOV", "BVV", "Fii", "Faa") and amplitudes
+ * T1[o,v], T2[o,o,v,v].  Every call is synchronous; the library copies its inputs and
+ * keeps no host pointer.  All calls return 0 on success and a negative code on failure
+ * (message in fh_last_error()); "not converged" is reported through the result struct
+ * with return code 0, as the reference does.
+ *
+ * Error codes: -1 invalid argument, -2 HIP runtime error, -3 internal/shape error,
+ *              -4 out of device memory, -5 non-finite amplitudes or energy,
+ *              -6 call made in the wrong order (e.g. no DF tensors set).
+ *
+ * A context is not re-entrant; separate contexts may be used from separate threads.
+ */
+#ifndef FERMIHIP_H
+#define FERMIHIP_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fh_ctx fh_ctx;
+
+/* Options of the CCSD iteration.  The first nine fields mirror Fermi's option keys
+ * (src/Core/Options.jl:62-102): cc_max_iter=50, cc_e_conv=1e-10, cc_max_rms=1e-10,
+ * cc_diis=1, cc_ndiis=3, diis_start=3, cc_diis_relax=3, diis_prec ("single" -> 1,
+ * "double" -> 0), cc_damp_ratio=0, precision_override=0. */
+typedef struct fh_cc_opts {
+  int cc_max_iter;
+  double cc_e_conv;
+  double cc_max_rms;
+  int cc_diis;
+  int cc_ndiis;
+  int diis_start;
+  int cc_diis_relax;
+  int diis_prec;          /* 1: DIIS error vectors stored in FP32 (reference default), 0: FP64 */
+  double cc_damp_ratio;
+  int precision_override;
+  int verbose;            /* 1: print the iteration table (RCCSDa.jl:177,223) to stdout */
+} fh_cc_opts;
+
+/* Result of fh_ccsd_solve: the scalar fields of Fermi's RCCSD{T} (RCCSD.jl:61-69),
+ * correlation energies only (add the reference energy on the host). */
+typedef struct fh_cc_result {
+  double guess_energy;    /* correlation energy of the starting amplitudes */
+  double correlation;     /* final CCSD correlation energy */
+  double e_conv;          /* last Delta E */
+  double t_conv;          /* last max(rms(dT1), rms(dT2)) */
+  int niter;
+  int converged;
+} fh_cc_result;
+
+/* Fill opts with Fermi's defaults. */
+void fh_default_opts(fh_cc_opts* opts);
+
+/* Create a context on HIP device `device`.  Replaces nothing in the reference: Fermi
+ * keeps its state in the IntegralHelper cache (src/Core/Integrals/IntegralHelper.jl:50-56). */
+int fh_create(fh_ctx** ctx, int device);
+
+/* Hand over the DF tensors and orbital energies (copied to HBM).
+ * eps_o[o] = moints["Fii"], eps_v[v] = moints["Faa"] (Fock.jl:93-94);
+ * Boo[Q,o,o], Bov[Q,o,v], Bvv[Q,v,v] = moints["BOO"/"BOV"/"BVV"] (DFERI.jl:15-69).
+ * Replaces the compute! calls of RCCSD(moints, aoints, ::RCCSDa), RCCSDa.jl:37-44. */
+int fh_set_df(fh_ctx* ctx, int o, int v, int naux, const double* eps_o, const double* eps_v, const double* Boo,
+              const double* Bov, const double* Bvv);
+
+/* Assemble the 4-index integral blocks on the device (DFERI.jl:71-197).  Optional: the
+ * other entry points call it on first use. */
+int fh_prepare(fh_ctx* ctx);
+
+/* Initial amplitudes.  kind 0: MP2 guess T1 = 0, T2 = (ia|jb)/D2 (RCCSDa.jl:46-90).
+ * T1[o,v], T2[o,o,v,v] are host outputs (either may be NULL); the guess is also kept on
+ * the device as the starting point of fh_ccsd_solve with NULL amplitudes. */
+int fh_t2_guess(fh_ctx* ctx, int kind, double* T1, double* T2);
+
+/* cc_update_energy (RCCSDHelper.jl:8-23) of host amplitudes (NULL: device-resident ones). */
+int fh_ccsd_energy(fh_ctx* ctx, const double* T1, const double* T2, double* Ecorr);
+
+/* One update_amp! (RCCSDHelper.jl:304-349): newT = (T1/T2 residual numerators) ./ D. */
+int fh_ccsd_residual(fh_ctx* ctx, const double* T1, const double* T2, double* newT1, double* newT2);
+
+/* The CCSD iteration RCCSD(moints, newT1, newT2, ::RCCSDa), RCCSDa.jl:92-245.
+ * T1/T2: starting amplitudes on input (NULL: the device-resident MP2 guess), final
+ * amplitudes on output when non-NULL.  The final amplitudes stay on the device for
+ * fh_triples. */
+int fh_ccsd_solve(fh_ctx* ctx, const fh_cc_opts* opts, double* T1, double* T2, fh_cc_result* res);
+
+/* (T) correction RCCSDpT(ccsd, moints, ::ijk), ijk.jl:20-150.  prec_mode 0 = FP64.
+ * T1/T2 NULL: use the device-resident amplitudes of the last solve. */
+int fh_triples(fh_ctx* ctx, int prec_mode, const double* T1, const double* T2, double* Et);
+
+/* Per-phase timings, flop counts and the iteration table as JSON (NUL-terminated). */
+int fh_stats(fh_ctx* ctx, char* json, size_t cap);
+
+/* Message of the last failed call on this thread. */
+const char* fh_last_error(void);
+
+void fh_destroy(fh_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,321 @@
+"""CPU oracle (test infrastructure only): FP64 restatement of Fermi.jl's closed-shell
+DF-RCCSD algorithm ``RCCSDa``.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg may
+import this module, and only as the checker.  The product path (``libfermihip.so``)
+never calls it.
+
+Every function restates the reference term by term, with the same index names and
+array index order as the Julia source (numpy arrays here are indexed exactly like
+the Julia arrays, e.g. ``T2[i,j,a,b]``, ``Vovov[i,a,j,b] = (ia|jb)``).  Julia's
+``@tensor``/``@tensoropt`` pairwise contractions are restated with
+``numpy.einsum(..., optimize=True)`` (pairwise contractions over BLAS), which is
+the same evaluation strategy TensorOperations.jl uses.
+
+Pins (see ``tests/test_oracle_*.py``):
+  * ``cc_update_energy``, ``cc_update_T1``, ``cc_update_T2``, ``update_amp``: against a
+    brute-force spin-orbital CCSD written from the textbook equations
+    (``oracle/spinorb.py``) at random *unconverged* amplitudes, and against the
+    DF-MP2 closed form (T=0 -> one update -> MP2 amplitudes and energy).
+  * ``df_ladder``: equals the conventional ladder ``tau.VVVV`` (RCCSDHelper.jl:194-202).
+  * ``DIISManager``: 2-vector Pulay closed form, BoundsError with <2 vectors
+    (test/test_diis.jl:1-9), deterministic largest-error replacement.
+  * ``rccsd_solve``: converged energy equals the spin-orbital CCSD fixed point.
+"""
+import numpy as np
+
+__all__ = [
+    "build_moints", "mp2_guess", "cc_update_energy", "cc_update_T1", "cc_update_T2",
+    "df_ladder", "conv_ladder", "update_amp", "DIISManager", "rccsd_solve", "Options",
+]
+
+
+def _ein(sub, *ops):
+    return np.einsum(sub, *ops, optimize=True)
+
+
+# Defaults of the CC keys read on the hot path: src/Core/Options.jl:62-102.
+def Options(**overrides):
+    opts = dict(
+        cc_max_iter=50, cc_e_conv=1e-10, cc_max_rms=1e-10, cc_diis=True, cc_ndiis=3,
+        diis_start=3, cc_diis_relax=3, diis_prec="single", cc_damp_ratio=0.0,
+        precision_override=False,
+    )
+    for k, v in overrides.items():
+        if k not in opts:
+            raise KeyError(f"unknown option {k}")
+        opts[k] = v
+    return opts
+
+
+def build_moints(Boo, Bov, Bvv, with_vvvv=True):
+    """4-index MO integrals assembled from the DF tensors (Chemist's notation).
+
+    src/Core/Integrals/ROIntegrals/DFERI.jl:71-197:
+      OOOO[i,j,k,l] = B[Q,i,j]B[Q,k,l]   (:83)      OOOV[i,j,k,a] = Boo[Q,i,j]Bov[Q,k,a] (:109)
+      OOVV[i,j,a,b] = Boo[Q,i,j]Bvv[Q,a,b] (:134)   OVOV[i,a,j,b] = Bov[Q,i,a]Bov[Q,j,b] (:151)
+      OVVV[i,a,b,c] = Bov[Q,i,a]Bvv[Q,b,c] (:177)   VVVV[a,b,c,d] = Bvv[Q,a,b]Bvv[Q,c,d] (:194)
+    """
+    td = lambda X, Y: np.tensordot(X, Y, axes=(0, 0))   # sum over Q, free indices in order
+    m = {
+        "BOO": Boo, "BOV": Bov, "BVV": Bvv,
+        "OOOO": td(Boo, Boo),
+        "OOOV": td(Boo, Bov),
+        "OOVV": td(Boo, Bvv),
+        "OVOV": td(Bov, Bov),
+        "OVVV": td(Bov, Bvv),
+    }
+    if with_vvvv:
+        m["VVVV"] = td(Bvv, Bvv)
+    return m
+
+
+def denominators(eps_o, eps_v):
+    """D1[i,a] = e_i - e_a, D2[i,j,a,b] = e_i + e_j - e_a - e_b (RCCSDa.jl:58-84)."""
+    d1 = eps_o[:, None] - eps_v[None, :]
+    d2 = (eps_o[:, None, None, None] + eps_o[None, :, None, None]
+          - eps_v[None, None, :, None] - eps_v[None, None, None, :])
+    return d1, d2
+
+
+def mp2_guess(moints, eps_o, eps_v):
+    """MP2 guess: T1 = Fia./D1 (Fia = 0 for RHF orbitals, Fock.jl:95),
+    T2 = permutedims(OVOV,(1,3,2,4))./D2  (RCCSDa.jl:53-87)."""
+    d1, d2 = denominators(eps_o, eps_v)
+    o, v = d1.shape
+    T1 = np.zeros((o, v)) / d1
+    T2 = np.transpose(moints["OVOV"], (0, 2, 1, 3)) / d2
+    return T1, T2
+
+
+def cc_update_energy(T1, T2, moints):
+    """RCCSDHelper.jl:8-23 (RHF orbitals, no off-diagonal Fock)."""
+    Vovov = moints["OVOV"]
+    B = 2.0 * np.transpose(T2, (1, 2, 0, 3))           # B[l,c,k,d] := 2*T2[k,l,c,d]   (:15)
+    B -= _ein("lc,kd->lckd", T1, T1)                   # -= T1[l,c]*T1[k,d]           (:16)
+    B -= np.transpose(T2, (0, 2, 1, 3))                # -= T2[l,k,c,d]               (:17)
+    E = _ein("lckd,kcld->", B, Vovov)                  # += B[l,c,k,d]*Vovov[k,c,l,d] (:18)
+    E += 2.0 * _ein("lc,kd,lckd->", T1, T1, Vovov)     # += 2*T1 T1 Vovov[l,c,k,d]    (:19)
+    return float(E)
+
+
+def cc_update_T1(T1, T2, moints):
+    """Numerator of the T1 update: RCCSDHelper.jl:57-85 (20 closed-shell terms)."""
+    Voovv, Vovov, Vovvv, Vooov = moints["OOVV"], moints["OVOV"], moints["OVVV"], moints["OOOV"]
+    e = _ein
+    n = np.zeros_like(T1)
+    n -= e("kc,ikca->ia", T1, Voovv)                                   # :64
+    n += 2 * e("kc,kcia->ia", T1, Vovov)                               # :65
+    n -= e("kicd,kdac->ia", T2, Vovvv)                                 # :66
+    n += 2 * e("ikcd,kdac->ia", T2, Vovvv)                             # :67
+    n += -2 * e("klac,kilc->ia", T2, Vooov)                            # :68
+    n += e("lkac,kilc->ia", T2, Vooov)                                 # :69
+    n += -2 * e("kc,la,likc->ia", T1, T1, Vooov)                       # :70
+    n -= e("kc,id,kdac->ia", T1, T1, Vovvv)                            # :71
+    n += 2 * e("kc,id,kcad->ia", T1, T1, Vovvv)                        # :72
+    n += e("kc,la,kilc->ia", T1, T1, Vooov)                            # :73
+    n += -2 * e("kc,ilad,lckd->ia", T1, T2, Vovov)                     # :74
+    n += -2 * e("kc,liad,kcld->ia", T1, T2, Vovov)                     # :75
+    n += e("kc,liad,lckd->ia", T1, T2, Vovov)                          # :76
+    n += -2 * e("ic,lkad,lckd->ia", T1, T2, Vovov)                     # :77
+    n += e("ic,lkad,kcld->ia", T1, T2, Vovov)                          # :78
+    n += -2 * e("la,ikdc,kcld->ia", T1, T2, Vovov)                     # :79
+    n += e("la,ikcd,kcld->ia", T1, T2, Vovov)                          # :80
+    n += e("kc,id,la,lckd->ia", T1, T1, T1, Vovov)                     # :81
+    n += -2 * e("kc,id,la,kcld->ia", T1, T1, T1, Vovov)                # :82
+    n += 4 * e("kc,ilad,kcld->ia", T1, T2, Vovov)                      # :83
+    return n
+
+
+def conv_ladder(T1, T2, moints):
+    """Conventional PP ladder: tau[i,j,a,b] = T2 + T1[i,a]T1[j,b];
+    newT2 += tau[i,j,c,d]*Vvvvv[c,a,d,b]  (RCCSDHelper.jl:194-202)."""
+    tau = T2 + _ein("ia,jb->ijab", T1, T1)
+    return _ein("ijcd,cadb->ijab", tau, moints["VVVV"])
+
+
+def df_ladder(T1, T2, Bvv):
+    """DF PP ladder, RCCSDHelper.jl:204-220: per virtual a,
+    cdb[c,d,b] = Bvv[Q,c,a]*Bvv[Q,d,b];  newT2[:,:,a,:] += tau[i,j,c,d]*cdb[c,d,b]."""
+    tau = T2 + _ein("ia,jb->ijab", T1, T1)             # :208
+    o, v = T1.shape
+    out = np.zeros_like(T2)
+    for a in range(v):                                 # :214
+        Ba = Bvv[:, :, a]                              # :215
+        cdb = _ein("Qc,Qdb->cdb", Ba, Bvv)             # :216
+        out[:, :, a, :] += _ein("ijcd,cdb->ijb", tau, cdb)   # :217-218
+    return out
+
+
+def cc_update_T2(T1, T2, moints, df=True):
+    """Numerator of the T2 update: RCCSDHelper.jl:123-192 (+ ladder :194-220)."""
+    Voooo, Vooov, Voovv = moints["OOOO"], moints["OOOV"], moints["OOVV"]
+    Vovov, Vovvv = moints["OVOV"], moints["OVVV"]
+    e = _ein
+    if df:
+        n = df_ladder(T1, T2, moints["BVV"])                           # :129 (DF twin :204-220)
+    else:
+        n = conv_ladder(T1, T2, moints)                                # :129 (conv :194-202)
+    n += np.transpose(Vovov, (0, 2, 1, 3))                             # :134 Vovov[i,a,j,b]
+    n += e("ka,lb,ikjl->ijab", T1, T1, Voooo)                          # :135
+    n += e("klab,ikjl->ijab", T2, Voooo)                               # :136
+    n -= e("ic,jd,ka,kcbd->ijab", T1, T1, T1, Vovvv)                   # :137
+    n -= e("ic,jd,kb,kdac->ijab", T1, T1, T1, Vovvv)                   # :138
+    n += e("ic,ka,lb,ljkc->ijab", T1, T1, T1, Vooov)                   # :139
+    n += e("jc,ka,lb,kilc->ijab", T1, T1, T1, Vooov)                   # :140
+    n += e("klac,ijdb,kcld->ijab", T2, T2, Vovov)                      # :141
+    n += -2 * e("ikac,ljbd,kcld->ijab", T2, T2, Vovov)                 # :142
+    n += -2 * e("lkac,ijdb,kcld->ijab", T2, T2, Vovov)                 # :143
+    n += e("kiac,ljdb,lckd->ijab", T2, T2, Vovov)                      # :144
+    n += e("ikac,ljbd,lckd->ijab", T2, T2, Vovov)                      # :145
+    n += -2 * e("ikac,jlbd,lckd->ijab", T2, T2, Vovov)                 # :146
+    n += e("kiac,ljbd,kcld->ijab", T2, T2, Vovov)                      # :147
+    n += -2 * e("kiac,jlbd,kcld->ijab", T2, T2, Vovov)                 # :148
+    n += e("ijac,lkbd,kcld->ijab", T2, T2, Vovov)                      # :149
+    n += -2 * e("ijac,klbd,kcld->ijab", T2, T2, Vovov)                 # :150
+    n += e("kjac,ildb,lckd->ijab", T2, T2, Vovov)                      # :151
+    n += 4 * e("ikac,jlbd,kcld->ijab", T2, T2, Vovov)                  # :152
+    n += e("ijdc,lkab,kcld->ijab", T2, T2, Vovov)                      # :153
+    n += e("ic,jd,ka,lb,kcld->ijab", T1, T1, T1, T1, Vovov)            # :154
+    n += e("ic,jd,lkab,lckd->ijab", T1, T1, T2, Vovov)                 # :155
+    n += e("ka,lb,ijdc,lckd->ijab", T1, T1, T2, Vovov)                 # :156
+    P = e("jc,iacb->ijab", T1, Vovvv)                                  # :157
+    P -= e("kb,jkia->ijab", T1, Vooov)                                 # :158
+    P -= e("kiac,kcjb->ijab", T2, Vovov)                               # :159
+    P -= e("ic,ka,kcjb->ijab", T1, T1, Vovov)                          # :160
+    P -= e("ic,kb,jkca->ijab", T1, T1, Voovv)                          # :161
+    P += 2 * e("ikac,kcjb->ijab", T2, Vovov)                           # :162
+    P -= e("ikac,jkcb->ijab", T2, Voovv)                               # :163
+    P -= e("kjac,ikcb->ijab", T2, Voovv)                               # :164
+    P += -2 * e("lb,ikac,ljkc->ijab", T1, T2, Vooov)                   # :165
+    P += e("lb,kiac,ljkc->ijab", T1, T2, Vooov)                        # :166
+    P -= e("jc,ikdb,kcad->ijab", T1, T2, Vovvv)                        # :167
+    P -= e("jc,kiad,kdbc->ijab", T1, T2, Vovvv)                        # :168
+    P -= e("jc,ikad,kcbd->ijab", T1, T2, Vovvv)                        # :169
+    P += e("jc,lkab,likc->ijab", T1, T2, Vooov)                        # :170
+    P += e("lb,ikac,kjlc->ijab", T1, T2, Vooov)                        # :171
+    P -= e("ka,ijdc,kdbc->ijab", T1, T2, Vovvv)                        # :172
+    P += e("ka,ilcb,ljkc->ijab", T1, T2, Vooov)                        # :173
+    P += 2 * e("jc,ikad,kdbc->ijab", T1, T2, Vovvv)                    # :174
+    P -= e("kc,ijad,kdbc->ijab", T1, T2, Vovvv)                        # :175
+    P += 2 * e("kc,ijad,kcbd->ijab", T1, T2, Vovvv)                    # :176
+    P += e("kc,ilab,kjlc->ijab", T1, T2, Vooov)                        # :177
+    P += -2 * e("kc,ilab,ljkc->ijab", T1, T2, Vooov)                   # :178
+    P += e("jkcd,ilab,kcld->ijab", T2, T2, Vovov)                      # :179
+    P += -2 * e("kc,jd,ilab,kcld->ijab", T1, T1, T2, Vovov)            # :180
+    P += e("kc,jd,ilab,lckd->ijab", T1, T1, T2, Vovov)                 # :181
+    P += -2 * e("kc,la,ijdb,kcld->ijab", T1, T1, T2, Vovov)            # :182
+    P += e("kc,la,ijdb,lckd->ijab", T1, T1, T2, Vovov)                 # :183
+    P += e("ic,ka,ljbd,kcld->ijab", T1, T1, T2, Vovov)                 # :184
+    P += -2 * e("ic,ka,jlbd,kcld->ijab", T1, T1, T2, Vovov)            # :185
+    P += e("ic,ka,ljdb,lckd->ijab", T1, T1, T2, Vovov)                 # :186
+    P += e("ic,lb,kjad,kcld->ijab", T1, T1, T2, Vovov)                 # :187
+    P += -2 * e("ikdc,ljab,kcld->ijab", T2, T2, Vovov)                 # :188
+    n += P + np.transpose(P, (1, 0, 3, 2))                             # :190 P[i,j,a,b]+P[j,i,b,a]
+    return n
+
+
+def update_amp(T1, T2, moints, eps_o, eps_v, df=True):
+    """RHF ``update_amp!``: RCCSDHelper.jl:304-349 -> (newT1, newT2)."""
+    d1, d2 = denominators(eps_o, eps_v)
+    newT1 = cc_update_T1(T1, T2, moints) / d1           # :312, :347
+    newT2 = cc_update_T2(T1, T2, moints, df=df) / d2    # :313, :348
+    return newT1, newT2
+
+
+class DIISManager:
+    """Pulay DIIS, src/Core/DIIS.jl:21-96.
+
+    ``vec_dtype``/``err_dtype`` play the roles of the type parameters T1/T2
+    (errors are stored in ``diis_prec``, FP32 by default, RCCSDa.jl:122-125)."""
+
+    def __init__(self, size=6, vec_dtype=np.float64, err_dtype=np.float32):
+        self.vecs, self.errs, self.max_vec = [], [], size
+        self.vec_dtype, self.err_dtype = vec_dtype, err_dtype
+
+    def __len__(self):
+        return len(self.vecs)
+
+    def push(self, V, E):
+        """DIIS.jl:38-50: when full, drop the vector whose error has the largest norm."""
+        if len(self) + 1 > self.max_vec:
+            norms = [np.linalg.norm(e) for e in self.errs]
+            idx = int(np.argmax(norms))                # findmax -> first maximum
+            del self.vecs[idx]
+            del self.errs[idx]
+        self.vecs.append(np.array(V, dtype=self.vec_dtype, copy=True))
+        self.errs.append(np.array(E, dtype=self.err_dtype, copy=True))
+
+    def extrapolate(self, add_res=False):
+        """DIIS.jl:59-96."""
+        if len(self) < 2:
+            raise IndexError("DIIS cannot extrapolate from one vector")   # BoundsError :61-63
+        N = len(self) + 1
+        B = np.ones((N, N), dtype=self.vec_dtype)
+        B[-1, -1] = 0.0
+        for n1, e1 in enumerate(self.errs):
+            for n2, e2 in enumerate(self.errs):
+                B[n1, n2] = np.sum(e1 * e2)            # computed in the error precision (:74)
+        resid = np.zeros(N, dtype=self.vec_dtype)
+        resid[-1] = 1.0
+        ci = np.linalg.solve(B, resid)                 # B\resid (:83)
+        out = sum(c * v for c, v in zip(ci[:-1], self.vecs))
+        if add_res:
+            out = out + sum(c * e.astype(self.vec_dtype) for c, e in zip(ci[:-1], self.errs))
+        return out
+
+
+def rccsd_solve(moints, eps_o, eps_v, T1, T2, opts=None, df=True, log=None):
+    """Iteration driver ``RCCSD(moints, newT1, newT2, ::RCCSDa)``: RCCSDa.jl:92-245.
+
+    Returns a dict with the fields of ``RCCSD{T}`` (RCCSD.jl:61-69) plus the iteration
+    history.  Non-convergence is not an error (RCCSDa.jl:179-182)."""
+    opts = Options() if opts is None else opts
+    o, v = T1.shape
+    if o < 1 or v < 1:
+        raise ValueError("invalid orbital space")
+    cc_e_conv, cc_max_rms = opts["cc_e_conv"], opts["cc_max_rms"]
+    epsT = np.finfo(np.float64).eps
+    if (cc_e_conv < epsT or cc_max_rms < epsT) and not opts["precision_override"]:
+        cc_e_conv = cc_max_rms = epsT                                   # :139-146
+    dp = opts["cc_damp_ratio"]
+    do_diis, ndiis = opts["cc_diis"], opts["cc_ndiis"]
+    diis_start, diis_relax = opts["diis_start"], opts["cc_diis_relax"]
+    err_dtype = {"double": np.float64, "single": np.float32, "half": np.float16}[opts["diis_prec"]]
+    DM1 = DIISManager(ndiis, np.float64, err_dtype) if do_diis else None   # :171-174
+    DM2 = DIISManager(ndiis, np.float64, err_dtype) if do_diis else None
+
+    newT1, newT2 = np.array(T1, dtype=np.float64), np.array(T2, dtype=np.float64)
+    Ecc = cc_update_energy(newT1, newT2, moints)                        # :160
+    Eguess = Ecc
+    dE, rms, ite = 1.0, 1.0, 1
+    history = []
+    while abs(dE) > cc_e_conv or rms > cc_max_rms:                      # :178
+        if ite > opts["cc_max_iter"]:
+            break                                                       # :179-182
+        T1c, T2c = newT1.copy(), newT2.copy()                           # :185-186
+        oldE = Ecc
+        newT1, newT2 = update_amp(T1c, T2c, moints, eps_o, eps_v, df=df)   # :190
+        r1 = np.sqrt(np.sum((newT1 - T1c) ** 2) / T1c.size)             # :193
+        r2 = np.sqrt(np.sum((newT2 - T2c) ** 2) / T2c.size)             # :194
+        diis = False
+        if do_diis:
+            DM1.push(newT1, newT1 - T1c)                                # :199-201
+            DM2.push(newT2, newT2 - T2c)                                # :200-202
+            if ite >= diis_start and ite % diis_relax == 0:             # :204
+                newT2 = DM2.extrapolate(add_res=True)                   # :206
+                newT1 = DM1.extrapolate(add_res=True)                   # :207
+                diis = True
+        if dp > 0:                                                      # :213-216
+            newT1 = (1 - dp) * newT1 + dp * T1c
+            newT2 = (1 - dp) * newT2 + dp * T2c
+        Ecc = cc_update_energy(newT1, newT2, moints)                    # :218
+        rms = max(r1, r2)                                               # :220
+        dE = Ecc - oldE                                                 # :221
+        history.append((ite, Ecc, dE, rms, diis))
+        if log is not None:
+            log(ite, Ecc, dE, rms, diis)
+        ite += 1
+    converged = abs(dE) < cc_e_conv and rms < cc_max_rms                # :231
+    return dict(guessenergy=Eguess, correlation=Ecc, T1=newT1, T2=newT2, e_conv=dE,
+                t_conv=rms, niter=ite - 1, converged=converged, history=history)

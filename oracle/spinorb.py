@@ -1,0 +1,167 @@
+"""Brute-force spin-orbital CCSD and (T) from the textbook equations (test
+infrastructure only; tiny sizes).
+
+This module is independent of Fermi.jl's closed-shell term list: it builds
+antisymmetrised spin-orbital integrals <pq||rs> from the spatial DF tensors and
+evaluates the spin-orbital CCSD equations of Stanton, Gauss, Watts & Bartlett,
+J. Chem. Phys. 94, 4334 (1991), in the form tabulated by Crawford & Schaefer,
+Rev. Comp. Chem. 14, 33 (2000), and the (T) correction of Raghavachari, Trucks,
+Pople & Head-Gordon, Chem. Phys. Lett. 157, 479 (1989) in its spin-orbital form
+E(T) = 1/36 sum t_c D (t_c + t_d).
+
+It pins the closed-shell restatement in ``oracle/rccsd.py`` and ``oracle/triples.py``:
+for a closed-shell (T1, T2) the spin-orbital amplitudes are
+  t_{I}^{A}   = delta(sI,sA) T1[i,a]
+  t_{IJ}^{AB} = delta(sI,sA)delta(sJ,sB) T2[i,j,a,b] - delta(sI,sB)delta(sJ,sA) T2[i,j,b,a]
+and the closed-shell numerators equal the (i alpha, a alpha) block of the
+spin-orbital T1 numerator and the (i alpha, j beta, a alpha, b beta) block of
+the T2 numerator.  Pure numpy; sizes o<=3, v<=5 run in well under a second.
+"""
+import numpy as np
+
+__all__ = ["SpinOrbitalCC", "closed_to_spin"]
+
+
+def _full_B(Boo, Bov, Bvv):
+    Q, o, _ = Boo.shape
+    v = Bvv.shape[1]
+    B = np.zeros((Q, o + v, o + v))
+    B[:, :o, :o] = Boo
+    B[:, :o, o:] = Bov
+    B[:, o:, :o] = np.transpose(Bov, (0, 2, 1))
+    B[:, o:, o:] = Bvv
+    return B
+
+
+def closed_to_spin(T1, T2):
+    """Closed-shell amplitudes -> spin-orbital amplitudes (occupied I=2i+s, virtual A=2a+s)."""
+    o, v = T1.shape
+    s_o = np.arange(2 * o) % 2
+    s_v = np.arange(2 * v) % 2
+    io = np.arange(2 * o) // 2
+    av = np.arange(2 * v) // 2
+    t1 = (s_o[:, None] == s_v[None, :]) * T1[io][:, av]
+    d = lambda x, y: (x == y).astype(np.float64)
+    SI, SJ, SA, SB = np.meshgrid(s_o, s_o, s_v, s_v, indexing="ij")
+    II, JJ, AA, BB = np.meshgrid(io, io, av, av, indexing="ij")
+    t2 = d(SI, SA) * d(SJ, SB) * T2[II, JJ, AA, BB] - d(SI, SB) * d(SJ, SA) * T2[II, JJ, BB, AA]
+    return t1, t2
+
+
+class SpinOrbitalCC:
+    """Spin-orbital CCSD/(T) for a closed-shell reference with Fock f = diag(eps)."""
+
+    def __init__(self, Boo, Bov, Bvv, eps_o, eps_v):
+        o, v = len(eps_o), len(eps_v)
+        self.o, self.v = o, v
+        B = _full_B(Boo, Bov, Bvv)
+        eri = np.einsum("Qpq,Qrs->pqrs", B, B)            # spatial (pq|rs)
+        n = o + v
+        # spin orbitals: occupied block first (2i+s), then virtual block (2a+s)
+        sp = np.concatenate([np.arange(o).repeat(2), (o + np.arange(v)).repeat(2)])
+        ss = np.tile([0, 1], n)
+        P, Qi, R, S = np.meshgrid(np.arange(2 * n), np.arange(2 * n), np.arange(2 * n), np.arange(2 * n), indexing="ij")
+        # <pq|rs> = (pr|qs) delta(sp,sr) delta(sq,ss)
+        phys = eri[sp[P], sp[R], sp[Qi], sp[S]] * (ss[P] == ss[R]) * (ss[Qi] == ss[S])
+        self.g = phys - np.transpose(phys, (0, 1, 3, 2))  # <pq||rs>
+        eps = np.concatenate([np.repeat(eps_o, 2), np.repeat(eps_v, 2)])
+        self.f = np.diag(eps)
+        self.no, self.nv = 2 * o, 2 * v
+        no = self.no
+        self.Dia = eps[:no, None] - eps[None, no:]
+        self.Dijab = (eps[:no, None, None, None] + eps[None, :no, None, None]
+                      - eps[None, None, no:, None] - eps[None, None, None, no:])
+
+    # slices
+    def _s(self):
+        no = self.no
+        return slice(0, no), slice(no, None)
+
+    def energy(self, t1, t2):
+        o, v = self._s()
+        g, f = self.g, self.f
+        return float(np.einsum("ia,ia->", f[o, v], t1) + 0.25 * np.einsum("ijab,ijab->", g[o, o, v, v], t2)
+                     + 0.5 * np.einsum("ijab,ia,jb->", g[o, o, v, v], t1, t1))
+
+    def numerators(self, t1, t2):
+        """Right-hand sides D*t1_new, D*t2_new (Crawford & Schaefer 2000, CCSD equations)."""
+        o, v = self._s()
+        g, f = self.g, self.f
+        e = lambda s, *x: np.einsum(s, *x, optimize=True)
+        no, nv = self.no, self.nv
+        tau_t = t2 + 0.5 * (e("ia,jb->ijab", t1, t1) - e("ib,ja->ijab", t1, t1))
+        tau = t2 + e("ia,jb->ijab", t1, t1) - e("ib,ja->ijab", t1, t1)
+        fvv, foo, fov = f[v, v], f[o, o], f[o, v]
+        Fae = (fvv - np.diag(np.diag(fvv))) - 0.5 * e("me,ma->ae", fov, t1) + e("mf,mafe->ae", t1, g[o, v, v, v]) \
+            - 0.5 * e("mnaf,mnef->ae", tau_t, g[o, o, v, v])
+        Fmi = (foo - np.diag(np.diag(foo))) + 0.5 * e("ie,me->mi", t1, fov) + e("ne,mnie->mi", t1, g[o, o, o, v]) \
+            + 0.5 * e("inef,mnef->mi", tau_t, g[o, o, v, v])
+        Fme = fov + e("nf,mnef->me", t1, g[o, o, v, v])
+        Wmnij = g[o, o, o, o] + e("je,mnie->mnij", t1, g[o, o, o, v]) - e("ie,mnje->mnij", t1, g[o, o, o, v]) \
+            + 0.25 * e("ijef,mnef->mnij", tau, g[o, o, v, v])
+        Wabef = g[v, v, v, v] - e("mb,amef->abef", t1, g[v, o, v, v]) + e("ma,bmef->abef", t1, g[v, o, v, v]) \
+            + 0.25 * e("mnab,mnef->abef", tau, g[o, o, v, v])
+        Wmbej = g[o, v, v, o] + e("jf,mbef->mbej", t1, g[o, v, v, v]) - e("nb,mnej->mbej", t1, g[o, o, v, o]) \
+            - e("jnfb,mnef->mbej", 0.5 * t2 + e("jf,nb->jnfb", t1, t1), g[o, o, v, v])
+        # T1
+        r1 = fov.copy() + e("ie,ae->ia", t1, Fae) - e("ma,mi->ia", t1, Fmi) + e("imae,me->ia", t2, Fme) \
+            - e("nf,naif->ia", t1, g[o, v, o, v]) - 0.5 * e("imef,maef->ia", t2, g[o, v, v, v]) \
+            - 0.5 * e("mnae,nmei->ia", t2, g[o, o, v, o])
+        # T2
+        r2 = g[o, o, v, v].copy()
+        X = Fae - 0.5 * e("mb,me->be", t1, Fme)                 # X[b,e]
+        tmp = e("ijae,be->ijab", t2, X)
+        r2 += tmp - np.transpose(tmp, (0, 1, 3, 2))
+        Y = Fmi + 0.5 * e("je,me->mj", t1, Fme)                 # Y[m,j]
+        tmp = e("imab,mj->ijab", t2, Y)
+        r2 -= tmp - np.transpose(tmp, (1, 0, 2, 3))
+        r2 += 0.5 * e("mnab,mnij->ijab", tau, Wmnij)
+        r2 += 0.5 * e("ijef,abef->ijab", tau, Wabef)
+        tmp = e("imae,mbej->ijab", t2, Wmbej) - e("ie,ma,mbej->ijab", t1, t1, g[o, v, v, o])
+        r2 += tmp - np.transpose(tmp, (1, 0, 2, 3)) - np.transpose(tmp, (0, 1, 3, 2)) + np.transpose(tmp, (1, 0, 3, 2))
+        tmp = e("ie,abej->ijab", t1, g[v, v, v, o])
+        r2 += tmp - np.transpose(tmp, (1, 0, 2, 3))
+        tmp = e("ma,mbij->ijab", t1, g[o, v, o, o])
+        r2 -= tmp - np.transpose(tmp, (0, 1, 3, 2))
+        return r1, r2
+
+    def update(self, t1, t2):
+        r1, r2 = self.numerators(t1, t2)
+        return r1 / self.Dia, r2 / self.Dijab
+
+    def solve(self, t1=None, t2=None, tol=1e-12, maxiter=500):
+        """Plain Jacobi iteration (no DIIS) to the CCSD fixed point."""
+        if t1 is None:
+            t1 = np.zeros((self.no, self.nv))
+            t2 = np.zeros((self.no, self.no, self.nv, self.nv))
+        E = self.energy(t1, t2)
+        for it in range(maxiter):
+            t1, t2 = self.update(t1, t2)
+            En = self.energy(t1, t2)
+            if abs(En - E) < tol:
+                return En, t1, t2, it + 1
+            E = En
+        raise RuntimeError("spin-orbital CCSD did not converge")
+
+    def triples(self, t1, t2):
+        """Spin-orbital (T): D t_c = P(i/jk)P(a/bc)[sum_e t_jk^ae <ei||bc> - sum_m t_im^bc <ma||jk>],
+        D t_d = P(i/jk)P(a/bc) t_i^a <jk||bc>, E = 1/36 sum t_c D (t_c + t_d)."""
+        o, v = self._s()
+        g = self.g
+        e = lambda s, *x: np.einsum(s, *x, optimize=True)
+        con = e("jkae,eibc->ijkabc", t2, g[v, o, v, v]) - e("imbc,majk->ijkabc", t2, g[o, v, o, o])
+        dis = e("ia,jkbc->ijkabc", t1, g[o, o, v, v])
+
+        def P3(x):
+            # P(i/jk) P(a/bc); P(i/jk) f(ijk) = f(ijk) - f(jik) - f(kji)
+            y = x - np.transpose(x, (1, 0, 2, 3, 4, 5)) - np.transpose(x, (2, 1, 0, 3, 4, 5))
+            return y - np.transpose(y, (0, 1, 2, 4, 3, 5)) - np.transpose(y, (0, 1, 2, 5, 4, 3))
+
+        eps = np.diag(self.f)
+        no = self.no
+        eo, ev = eps[:no], eps[no:]
+        D = (eo[:, None, None, None, None, None] + eo[None, :, None, None, None, None] + eo[None, None, :, None, None, None]
+             - ev[None, None, None, :, None, None] - ev[None, None, None, None, :, None] - ev[None, None, None, None, None, :])
+        tc = P3(con) / D
+        td = P3(dis) / D
+        return float(np.sum(tc * D * (tc + td)) / 36.0)

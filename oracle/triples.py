@@ -1,0 +1,159 @@
+"""CPU oracle (test infrastructure only): FP64 restatement of Fermi.jl's closed-shell
+(T) energy, all three reference algorithms.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg may
+import this module, and only as the checker.  The product path never calls it.
+
+* ``rccsdpt_ijk``  restates src/Methods/CoupledCluster/PerturbativeTriples/ijk.jl:20-150
+  (the default ``pt_alg=1``), with the reference's own permuted layouts.
+* ``rccsdpt_abc``  restates abc.jl:28-168 (o^3 intermediates per virtual triple).
+* ``rccsdpt_ijk2`` restates ijk2.jl:21-190 (explicit GEMMs + permute-and-accumulate).
+
+Pins (tests/test_oracle_triples.py): the three restatements agree with each other;
+brute-force spin-orbital (T) from the textbook equations (oracle/spinorb.py);
+E_T(lambda*T2) = lambda^2 E_T(T2) at T1 = 0; E_T(mu*integrals) = mu^2 E_T; invariance
+under relabelling orbitals within o and within v.
+"""
+import numpy as np
+
+__all__ = ["rccsdpt_ijk", "rccsdpt_abc", "rccsdpt_ijk2", "triple_list"]
+
+
+def _perm(A, p):
+    """Julia ``permutedims(A, p)`` with 1-based p (same semantics as numpy transpose)."""
+    return np.transpose(A, tuple(x - 1 for x in p))
+
+
+def triple_list(o):
+    """Occupied triples in the reference's loop order: for i, for j<=i, for k<=j (ijk.jl:49,63,83)."""
+    return [(i, j, k) for i in range(o) for j in range(i + 1) for k in range(j + 1)]
+
+
+def _abc_lower(v):
+    a, b, c = np.meshgrid(np.arange(v), np.arange(v), np.arange(v), indexing="ij")
+    m = (b <= a) & (c <= b)
+    return a[m], b[m], c[m]
+
+
+def _ijk_energy_block(W, V, Dijk, fv, dij, djk, idx):
+    """Energy loop over a>=b>=c of ijk.jl:120-136 (vectorised; same arithmetic per element)."""
+    a, b, c = idx
+    Dd = Dijk - fv[a] - fv[b] - fv[c]
+    dab = (a == b).astype(np.float64)
+    dbc = (b == c).astype(np.float64)
+    Wabc, Wacb, Wbac = W[a, b, c], W[a, c, b], W[b, a, c]
+    Wbca, Wcab, Wcba = W[b, c, a], W[c, a, b], W[c, b, a]
+    Vabc, Vacb, Vbac = V[a, b, c], V[a, c, b], V[b, a, c]
+    Vbca, Vcab, Vcba = V[b, c, a], V[c, a, b], V[c, b, a]
+    X = Wabc * Vabc + Wacb * Vacb + Wbac * Vbac + Wbca * Vbca + Wcab * Vcab + Wcba * Vcba   # :129
+    Y = Vabc + Vbca + Vcab                                                                  # :130
+    Z = Vacb + Vbac + Vcba                                                                  # :131
+    Ef = (Y - 2 * Z) * (Wabc + Wbca + Wcab) + (Z - 2 * Y) * (Wacb + Wbac + Wcba) + 3 * X     # :132
+    return float(np.sum(Ef * (2 - dij - djk) / (Dd * (1 + dab + dbc))))                     # :133
+
+
+def rccsdpt_ijk(T1, T2, moints, fo, fv, triples=None):
+    """(T) correction, ijk.jl:20-150.  ``triples`` optionally restricts the (i>=j>=k)
+    loop to a subset (used to time a bounded sample); default: all triples."""
+    T1p = _perm(T1, (2, 1))                       # T1p[a,i]            :24
+    T2p = _perm(T2, (4, 3, 2, 1))                 # T2p[b,a,j,i]=T2[i,j,a,b]  :25
+    Vvvvo = _perm(moints["OVVV"], (4, 3, 2, 1))   # Vvvvo[c,b,a,i]=(ia|bc)    :28
+    Vvooo = _perm(moints["OOOV"], (4, 1, 2, 3))   # Vvooo[a,i,j,k]=(ij|ka)    :31
+    Vvvoo = _perm(moints["OVOV"], (2, 4, 1, 3))   # Vvvoo[a,b,i,j]=(ia|jb)    :32
+    v, o = T1p.shape
+    idx = _abc_lower(v)
+    e = lambda s, x, y: np.einsum(s, x, y, optimize=True)
+    Et = 0.0
+    for (i, j, k) in (triple_list(o) if triples is None else triples):
+        Dijk = fo[i] + fo[j] + fo[k]
+        T2_i, T2_j, T2_k = T2p[:, :, :, i], T2p[:, :, :, j], T2p[:, :, :, k]
+        W = e("dba,dc->abc", Vvvvo[:, :, :, i], T2p[:, :, j, k]) - e("bl,acl->abc", Vvooo[:, :, i, j], T2_k)  # :109
+        W += e("dab,dc->abc", Vvvvo[:, :, :, j], T2p[:, :, i, k]) - e("al,bcl->abc", Vvooo[:, :, j, i], T2_k)  # :110
+        W += e("dac,db->abc", Vvvvo[:, :, :, k], T2p[:, :, i, j]) - e("al,cbl->abc", Vvooo[:, :, k, i], T2_j)  # :111
+        W += e("dbc,ad->abc", Vvvvo[:, :, :, k], T2p[:, :, i, j]) - e("bl,cal->abc", Vvooo[:, :, k, j], T2_i)  # :112
+        W += e("dca,bd->abc", Vvvvo[:, :, :, i], T2p[:, :, j, k]) - e("cl,abl->abc", Vvooo[:, :, i, k], T2_j)  # :113
+        W += e("dcb,ad->abc", Vvvvo[:, :, :, j], T2p[:, :, i, k]) - e("cl,bal->abc", Vvooo[:, :, j, k], T2_i)  # :114
+        V = (W + np.einsum("a,bc->abc", T1p[:, i], Vvvoo[:, :, j, k])
+             + np.einsum("ac,b->abc", Vvvoo[:, :, i, k], T1p[:, j])
+             + np.einsum("ab,c->abc", Vvvoo[:, :, i, j], T1p[:, k]))                                          # :116
+        Et += _ijk_energy_block(W, V, Dijk, fv, float(i == j), float(j == k), idx)
+    return Et
+
+
+def rccsdpt_abc(T1, T2, moints, fo, fv, faithful=True):
+    """(T) correction, abc.jl:28-168 (loop a>=b>=c, o^3 intermediates).
+
+    abc.jl:128 and :131 contract ``T2_c`` where the (jki)(bca) and (kji)(cba)
+    permutations of the first term (abc.jl:124) call for ``T2_a``.  ``faithful=True``
+    keeps the reference's operands; ``faithful=False`` uses ``T2_a``, which is the
+    form that agrees with ijk.jl and ijk2.jl."""
+    ovvv = _perm(moints["OVVV"], (1, 3, 2, 4))    # ovvv[i,b,a,c] = (ia|bc)   :37
+    ooov = _perm(moints["OOOV"], (1, 3, 2, 4))    # ooov[i,k,j,a] = (ij|ka)   :38
+    oovv = _perm(moints["OVOV"], (1, 3, 2, 4))    # oovv[i,j,a,b] = (ia|jb)   :39
+    o, v = T1.shape
+    ii, jj, kk = np.meshgrid(np.arange(o), np.arange(o), np.arange(o), indexing="ij")
+    m = (jj <= ii) & (kk <= jj)
+    ii, jj, kk = ii[m], jj[m], kk[m]
+    dij = (ii == jj).astype(np.float64)
+    djk = (jj == kk).astype(np.float64)
+    e = lambda s, x, y: np.einsum(s, x, y, optimize=True)
+    Et = 0.0
+    for a in range(v):
+        for b in range(a + 1):
+            for c in range(b + 1):
+                Dabc = fv[a] + fv[b] + fv[c]
+                W = e("id,jkd->ijk", ovvv[:, :, a, b], T2[:, :, :, c]) - e("jkl,il->ijk", ooov[:, :, :, c], T2[:, :, a, b])   # :124
+                W += e("id,kjd->ijk", ovvv[:, :, a, c], T2[:, :, :, b]) - e("kjl,il->ijk", ooov[:, :, :, b], T2[:, :, a, c])  # :125
+                W += e("jd,ikd->ijk", ovvv[:, :, b, a], T2[:, :, :, c]) - e("ikl,jl->ijk", ooov[:, :, :, c], T2[:, :, b, a])  # :127
+                T2x = T2[:, :, :, c] if faithful else T2[:, :, :, a]
+                W += e("jd,kid->ijk", ovvv[:, :, b, c], T2x) - e("kil,jl->ijk", ooov[:, :, :, a], T2[:, :, b, c])  # :128
+                W += e("kd,ijd->ijk", ovvv[:, :, c, a], T2[:, :, :, b]) - e("ijl,kl->ijk", ooov[:, :, :, b], T2[:, :, c, a])  # :130
+                W += e("kd,jid->ijk", ovvv[:, :, c, b], T2x) - e("jil,kl->ijk", ooov[:, :, :, a], T2[:, :, c, b])  # :131
+                V = (W + np.einsum("jk,i->ijk", oovv[:, :, b, c], T1[:, a])
+                     + np.einsum("ik,j->ijk", oovv[:, :, a, c], T1[:, b])
+                     + np.einsum("ij,k->ijk", oovv[:, :, a, b], T1[:, c]))                                                     # :133
+                dab, dbc = float(a == b), float(b == c)
+                i, j, k = ii, jj, kk
+                Dd = fo[i] + fo[j] + fo[k] - Dabc                                                                               # :139-144
+                X = (W[i, j, k] * V[i, j, k] + W[i, k, j] * V[i, k, j] + W[j, i, k] * V[j, i, k]
+                     + W[j, k, i] * V[j, k, i] + W[k, i, j] * V[k, i, j] + W[k, j, i] * V[k, j, i])                             # :146
+                Y = V[i, j, k] + V[j, k, i] + V[k, i, j]
+                Z = V[i, k, j] + V[j, i, k] + V[k, j, i]
+                Ef = (Y - 2 * Z) * (W[i, j, k] + W[k, i, j] + W[j, k, i])
+                Ef += (Z - 2 * Y) * (W[i, k, j] + W[j, i, k] + W[k, j, i]) + 3 * X                                             # :149-150
+                Et += float(np.sum(2 * Ef / (Dd * (1 + dij + djk) * (1 + dab + dbc))))                                          # :151
+    return Et
+
+
+def rccsdpt_ijk2(T1, T2, moints, fo, fv):
+    """(T) correction, ijk2.jl:21-190 (W as explicit GEMMs, permute-and-accumulate)."""
+    T1p = _perm(T1, (2, 1))
+    T2p = _perm(T2, (4, 3, 2, 1))
+    v, o = T1p.shape
+    ovoo = _perm(moints["OOOV"], (1, 4, 2, 3))                              # ovoo[l,b,i,j] = (li|jb)  :31
+    vvoo = _perm(moints["OVOV"], (2, 4, 1, 3))                              # :32
+    vvvo = _perm(moints["OVVV"], (3, 2, 4, 1)).reshape((v * v, v, o), order="F")   # :33 (column-major reshape)
+    idx = _abc_lower(v)
+
+    def gemm_pair(vvvo_x, T2_yz, T2_w, ovoo_yz):
+        # matmul(W, vvvo_x, T2_yz); matmul(W, T2_w, ovoo_yz, -1, 1)  (:111-112)
+        M = vvvo_x @ T2_yz - T2_w @ ovoo_yz
+        return M.reshape((v, v, v), order="F")
+
+    def T2cols(x):
+        return T2p[:, :, :, x].reshape((v * v, o), order="F")
+
+    Et = 0.0
+    for (i, j, k) in triple_list(o):
+        Dijk = fo[i] + fo[j] + fo[k]
+        W = gemm_pair(vvvo[:, :, j], T2p[:, :, i, k], T2cols(j), ovoo[:, :, i, k])                       # abc :110-113
+        W = W + _perm(gemm_pair(vvvo[:, :, k], T2p[:, :, i, j], T2cols(k), ovoo[:, :, i, j]), (1, 3, 2))  # acb :115-120
+        W = W + _perm(gemm_pair(vvvo[:, :, i], T2p[:, :, j, k], T2cols(i), ovoo[:, :, j, k]), (2, 1, 3))  # bac :122-127
+        W = W + _perm(gemm_pair(vvvo[:, :, k], T2p[:, :, j, i], T2cols(k), ovoo[:, :, j, i]), (3, 1, 2))  # bca :129-134
+        W = W + _perm(gemm_pair(vvvo[:, :, i], T2p[:, :, k, j], T2cols(i), ovoo[:, :, k, j]), (2, 3, 1))  # cab :136-141
+        W = W + _perm(gemm_pair(vvvo[:, :, j], T2p[:, :, k, i], T2cols(j), ovoo[:, :, k, i]), (3, 2, 1))  # cba :143-148
+        V = (W + np.einsum("a,bc->abc", T1p[:, i], vvoo[:, :, j, k])
+             + np.einsum("ac,b->abc", vvoo[:, :, i, k], T1p[:, j])
+             + np.einsum("ab,c->abc", vvoo[:, :, i, j], T1p[:, k]))                                       # :151-153
+        Et += _ijk_energy_block(W, V, Dijk, fv, float(i == j), float(j == k), idx)
+    return Et

@@ -1,0 +1,153 @@
+"""GPU parity of the HIP DF-RCCSD(T) path (through the C ABI) against the CPU oracle.
+
+FP64 path: residual numerators, energies and (T) must agree with the oracle to 1e-12
+relative (the only differences are FP64 summation order); converged energies to 1e-10 Eh.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import rccsd as R
+from oracle import triples as TT
+from helpers import random_df, random_amplitudes, relerr
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def ctx_for(eo, ev, Boo, Bov, Bvv):
+    import fermihip
+    c = fermihip.Context(0)
+    c.set_df(eo, ev, Boo, Bov, Bvv)
+    return c
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("o,v,Q,seed", [(1, 1, 3, 0), (2, 3, 7, 1), (3, 7, 11, 2), (5, 19, 84, 3), (8, 30, 50, 4),
+                                        (13, 17, 40, 5)])
+def test_residual_matches_oracle(o, v, Q, seed):
+    say(f"residual o={o} v={v} Q={Q} seed={seed}")
+    eo, ev, Boo, Bov, Bvv = random_df(o, v, Q, seed)
+    m = R.build_moints(Boo, Bov, Bvv)
+    T1, T2 = random_amplitudes(o, v, seed)
+    c = ctx_for(eo, ev, Boo, Bov, Bvv)
+    g1, g2 = c.update_amp(T1, T2)
+    r1, r2 = R.update_amp(T1, T2, m, eo, ev)
+    assert relerr(g1, r1) < 1e-12
+    assert relerr(g2, r2) < 1e-12
+    e_gpu, e_ref = c.energy(T1, T2), R.cc_update_energy(T1, T2, m)
+    assert abs(e_gpu - e_ref) < 1e-12 * max(1.0, abs(e_ref))
+
+
+@pytest.mark.parametrize("name", ["tiny_o2v3", "tiny_o3v4"])
+def test_golden(name):
+    say(f"golden {name}")
+    g = np.load(os.path.join(GOLDEN, name + ".npz"))
+    import fermihip
+    c = ctx_for(g["eps_o"], g["eps_v"], g["Boo"], g["Bov"], g["Bvv"])
+    d1 = g["eps_o"][:, None] - g["eps_v"][None, :]
+    d2 = (g["eps_o"][:, None, None, None] + g["eps_o"][None, :, None, None] - g["eps_v"][None, None, :, None]
+          - g["eps_v"][None, None, None, :])
+    n1, n2 = c.update_amp(g["T1"], g["T2"])
+    assert relerr(n1 * d1, g["num1"]) < 1e-12
+    assert relerr(n2 * d2, g["num2"]) < 1e-12
+    assert abs(c.energy(g["T1"], g["T2"]) - g["E_rand"]) < 1e-12
+    T1, T2 = c.guess()
+    assert abs(c.energy(T1, T2) - g["E_mp2"]) < 1e-13
+    r, t1, t2 = c.solve(fermihip.Options(diis_prec="double"), T1, T2)
+    assert r.converged == 1
+    assert abs(r.correlation - g["E_ccsd"]) < 1e-10
+    Et = c.triples(g["T1_ccsd"], g["T2_ccsd"])
+    assert abs(Et - g["Et_ccsd"]) < 1e-12
+    Et = c.triples(g["T1"], g["T2"])
+    assert abs(Et - g["Et_rand"]) < 1e-12 * abs(g["Et_rand"])
+
+
+@pytest.mark.parametrize("o,v,Q,seed,prec", [(3, 6, 12, 4, "double"), (5, 19, 84, 6, "double"), (5, 19, 84, 6, "single")])
+def test_solve_trajectory_matches_oracle(o, v, Q, seed, prec):
+    say(f"solve o={o} v={v} Q={Q} diis_prec={prec}")
+    import fermihip
+    from fermihip import synth
+    eo, ev, Boo, Bov, Bvv = synth.molecular_like(o, v, Q, seed)
+    m = R.build_moints(Boo, Bov, Bvv)
+    T1, T2 = R.mp2_guess(m, eo, ev)
+    ref = R.rccsd_solve(m, eo, ev, T1, T2, R.Options(diis_prec=prec))
+    c = ctx_for(eo, ev, Boo, Bov, Bvv)
+    r, t1, t2 = c.solve(fermihip.Options(diis_prec=prec), T1, T2)
+    assert r.converged == 1 and ref["converged"]
+    assert abs(r.correlation - ref["correlation"]) < 1e-10
+    assert abs(r.guess_energy - ref["guessenergy"]) < 1e-13
+    if prec == "double":
+        # FP64 DIIS: the iteration trajectory is the reference's
+        st = c.stats()["iterations"]
+        assert len(st) == ref["niter"] == r.niter
+        for row, h in zip(st, ref["history"]):
+            assert abs(row[1] - h[1]) < 1e-11
+            assert row[5] == h[4]
+        assert relerr(t2, ref["T2"]) < 1e-8
+
+
+@pytest.mark.parametrize("o,v,Q,seed,t1s", [(2, 3, 6, 0, 0.1), (3, 7, 9, 1, 0.1), (5, 19, 84, 2, 0.05), (6, 25, 40, 3, 0.0),
+                                            (7, 12, 30, 4, 0.2)])
+def test_triples_matches_oracle(o, v, Q, seed, t1s):
+    say(f"triples o={o} v={v} Q={Q} seed={seed}")
+    eo, ev, Boo, Bov, Bvv = random_df(o, v, Q, seed)
+    m = R.build_moints(Boo, Bov, Bvv)
+    T1, T2 = random_amplitudes(o, v, seed, s1=t1s)
+    c = ctx_for(eo, ev, Boo, Bov, Bvv)
+    e_gpu = c.triples(T1, T2)
+    e_ref = TT.rccsdpt_ijk(T1, T2, m, eo, ev)
+    assert abs(e_gpu - e_ref) < 1e-12 * abs(e_ref)
+
+
+def test_full_size_properties_water4():
+    """o=16, v=76, Q=336: MP2 closed form, closed-shell symmetry of the update, (T) scaling law,
+    bitwise determinism of repeated calls."""
+    say("properties water4 (16,76,336)")
+    from fermihip import synth
+    o, v, Q = synth.SHAPES["water4"]
+    eo, ev, Boo, Bov, Bvv = synth.molecular_like(o, v, Q, 1)
+    c = ctx_for(eo, ev, Boo, Bov, Bvv)
+    T1, T2 = c.guess()
+    iajb = np.einsum("Qia,Qjb->iajb", Bov, Bov, optimize=True)
+    D = eo[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] - ev[None, None, None, :]
+    Emp2 = np.sum(iajb * (2 * iajb - np.transpose(iajb, (0, 3, 2, 1))) / D)
+    assert abs(c.energy(T1, T2) - Emp2) < 1e-12 * abs(Emp2)
+    rng = np.random.default_rng(3)
+    A1 = T1 + 0.01 * rng.standard_normal(T1.shape)
+    n1, n2 = c.update_amp(A1, T2)
+    assert relerr(n2, np.transpose(n2, (1, 0, 3, 2))) < 1e-13
+    m1, m2 = c.update_amp(A1, T2)
+    assert np.array_equal(n1, m1) and np.array_equal(n2, m2)
+    Z = np.zeros_like(T1)
+    e1 = c.triples(Z, T2)
+    e2 = c.triples(Z, 0.5 * T2)
+    assert abs(e2 - 0.25 * e1) < 1e-12 * abs(e1)
+    assert c.triples(Z, T2) == e1
+
+
+def test_edge_cases_and_errors():
+    say("edge cases")
+    import fermihip
+    eo, ev, Boo, Bov, Bvv = random_df(2, 4, 8, 2, 0.1)
+    c = fermihip.Context(0)
+    with pytest.raises(fermihip.FermiException, match="fh_set_df"):
+        c.solve(fermihip.Options(), np.zeros((2, 4)), np.zeros((2, 2, 4, 4)))
+    with pytest.raises(fermihip.FermiException):
+        c.set_df(eo[::-1] + 5.0, ev, Boo, Bov, Bvv)   # occupied above virtual
+    c.set_df(eo, ev, Boo, Bov, Bvv)
+    T1, T2 = c.guess()
+    r, _, _ = c.solve(fermihip.Options(cc_max_iter=2), T1, T2)
+    assert r.converged == 0 and r.niter == 2          # not an error (RCCSDa.jl:179-182)
+    with pytest.raises(fermihip.FermiException):
+        c.triples(T1, T2, prec_mode=7)
+    bad = T2.copy()
+    bad[0, 0, 0, 0] = np.nan
+    with pytest.raises(fermihip.FermiException):
+        c.energy(T1, bad)
