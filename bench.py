@@ -163,7 +163,7 @@ def main():
         return {"tflops": p["flops"] / (p["ms"] * 1e-3) / 1e12, "ms": p["ms"] / max(1, args.steps),
                 "calls": p["calls"] // max(1, args.steps), "flops": p["flops"] / max(1, args.steps)}
 
-    kern = {n: rate(n) for n in ("ladder", "triples_gemm", "ring", "ladder_build", "integrals")}
+    kern = {n: rate(n) for n in ph}
     dom = max(("ladder", "triples_gemm"), key=lambda n: (kern[n]["ms"] if kern[n] else -1))
     kd = kern[dom]
     roof = {"kernel": {"ladder": "k_gemm_f64<128,128,16,false,false,1,1> (PP ladder)",

@@ -12,6 +12,8 @@
 // then run on the same XCD and hit its L2.
 #include "fh_internal.h"
 
+#include <algorithm>
+
 namespace fh {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -20,7 +22,7 @@ template <int BM, int BN, int BK, bool TA, bool TB, int OCC, int TAG>
 __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
     int64_t M, int64_t N, int64_t K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
     const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
-    int64_t sC, double* D, int64_t ldd, int64_t sD, int tilesM, int tilesN) {
+    int64_t sC, double* D, int64_t ldd, int64_t sD, int tilesM, int tilesN, int64_t kchunk, int nfast) {
   constexpr int PADA = 16, PADB = 2;
   constexpr int WM = BM / 2, WN = BN / 2, MB = WM / 16, NB = WN / 16;
   constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
@@ -35,12 +37,22 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
   const int64_t w = blockIdx.x;
   const int64_t q = nwg >> 3, r = nwg & 7, x = w & 7;
   const int64_t L = x * q + (x < r ? x : r) + (w >> 3);
-  const int64_t mt = L % tilesM, nt = L / tilesM;
+  // raster: M-tile fastest (tiles sharing a column block of B are neighbours) or, when A is
+  // the larger operand, N-tile fastest (a row block of A is reused by all N-tiles at once)
+  const int64_t mt = nfast ? L / tilesN : L % tilesM, nt = nfast ? L % tilesN : L / tilesM;
   const int64_t m0 = mt * BM, n0 = nt * BN;
 
   const int64_t bz = blockIdx.y;
-  A += bz * sA;
-  B += bz * sB;
+  if (kchunk > 0) {
+    // split-K: slice bz covers k in [bz*kchunk, min(K, (bz+1)*kchunk)) and writes its own D slice
+    const int64_t kb = bz * kchunk;
+    K = (K - kb < kchunk) ? K - kb : kchunk;
+    A += TA ? kb : lda * kb;
+    B += TB ? ldb * kb : kb;
+  } else {
+    A += bz * sA;
+    B += bz * sB;
+  }
   if (C) C += bz * sC;
   D += bz * sD;
 
@@ -132,42 +144,83 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
     }
 }
 
-template <int BM, int BN, bool TA, bool TB, int TAG>
+template <int BM, int BN, int BK, bool TA, bool TB, int TAG>
 static void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                    int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
-                   int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch) {
+                   int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk) {
   const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   FH_CHECK(tm * tn < (int64_t)1 << 31 && batch < 65536, -3, "gemm grid too large");
   dim3 grid((unsigned)(tm * tn), (unsigned)batch);
-  hipLaunchKernelGGL((k_gemm_f64<BM, BN, 16, TA, TB, (BM * BN > 8192 ? 1 : 2), TAG>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB,
-                     beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn);
+  const int nfast = (M > N) ? 1 : 0;   // stream the larger of A (M x K) and B (K x N) once
+  hipLaunchKernelGGL((k_gemm_f64<BM, BN, BK, TA, TB, (BM * BN > 8192 ? 1 : 4), TAG>), grid, dim3(256), 0, s, M, N, K, alpha, A,
+                     lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, nfast);
   FH_HIP(hipGetLastError());
 }
 
 template <bool TA, bool TB, int TAG>
 static void pick(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                  int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
-                 int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch) {
-  const int64_t big = ((M + 127) / 128) * ((N + 127) / 128) * batch;
-  if (big >= 512)
-    launch<128, 128, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch);
+                 int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk) {
+  // 64x64 tiles at up to four workgroups per CU measured fastest on every shape of this
+  // engine (the latency of the next operand tile is hidden by the other workgroups);
+  // BK = 8 when the per-tile K loop is short.
+  const int64_t kl = kchunk > 0 ? kchunk : K;
+  if (kl <= 512)
+    launch<64, 64, 8, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk);
   else
-    launch<64, 64, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch);
+    launch<64, 64, 16, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk);
+}
+
+// split-K sizing: few output tiles and a long K leave most CUs idle; cut K into slices
+static int64_t splitk_slices(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  if (batch != 1 || K < 4096) return 1;
+  const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  if (tiles >= 256) return 1;
+  int64_t sl = (1024 + tiles - 1) / tiles;
+  sl = std::min<int64_t>(sl, K / 512);
+  return std::max<int64_t>(sl, 1);
+}
+
+int64_t gemm_splitk_elems(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  const int64_t sl = splitk_slices(M, N, K, batch);
+  return sl > 1 ? sl * M * N : 0;
+}
+
+__global__ void k_splitk_reduce(int64_t M, int64_t N, int64_t S, const double* __restrict__ W, double alpha, double beta,
+                                const double* C, int64_t ldc, double* D, int64_t ldd) {
+  const int64_t n = M * N;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * 256) {
+    const int64_t m = idx % M, c = idx / M;
+    double acc = 0.0;
+    for (int64_t q = 0; q < S; ++q) acc += W[idx + q * n];
+    double val = alpha * acc;
+    if (beta != 0.0) val += beta * C[m + ldc * c];
+    D[m + ldd * c] = val;
+  }
 }
 
 void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
               int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C,
-              int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int tag) {
+              int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int tag, double* work,
+              int64_t work_elems) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   FH_CHECK(K >= 0, -3, "gemm: negative K");
   FH_CHECK(lda >= (ta ? (K > 0 ? K : 1) : M) && ldb >= (tb ? N : (K > 0 ? K : 1)) && ldd >= M, -3,
            "gemm: leading dimension too small");
   FH_CHECK(beta == 0.0 || (C != nullptr && ldc >= M), -3, "gemm: C missing");
-#define FH_PICK(TA_, TB_)                                                                                   \
-  do {                                                                                                     \
-    if (tag == 1) pick<TA_, TB_, 1>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch); \
-    else if (tag == 2) pick<TA_, TB_, 2>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch); \
-    else pick<TA_, TB_, 0>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch); \
+  const int64_t sl = splitk_slices(M, N, K, batch);
+  const bool split = sl > 1 && work != nullptr && work_elems >= sl * M * N;
+  const int64_t kchunk = split ? (K + sl - 1) / sl : 0;
+  const int64_t nb = split ? (K + kchunk - 1) / kchunk : batch;
+  const double al = split ? 1.0 : alpha, be = split ? 0.0 : beta;
+  const double* Cc = split ? nullptr : C;
+  double* Dd = split ? work : D;
+  const int64_t lddd = split ? M : ldd, sDD = split ? M * N : sD;
+#define FH_PICK(TA_, TB_)                                                                                            \
+  do {                                                                                                              \
+    if (tag == 1) pick<TA_, TB_, 1>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk); \
+    else if (tag == 2) pick<TA_, TB_, 2>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk); \
+    else pick<TA_, TB_, 0>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk);         \
   } while (0)
   if (ta) {
     if (tb) FH_PICK(true, true); else FH_PICK(true, false);
@@ -175,6 +228,12 @@ void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, 
     if (tb) FH_PICK(false, true); else FH_PICK(false, false);
   }
 #undef FH_PICK
+  if (split) {
+    int64_t g = (M * N + 255) / 256;
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)g), dim3(256), 0, s, M, N, nb, work, alpha, beta, C, ldc, D, ldd);
+    FH_HIP(hipGetLastError());
+  }
 }
 
 }  // namespace fh

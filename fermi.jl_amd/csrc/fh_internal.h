@@ -86,7 +86,9 @@ inline TView tv(double* p, std::initializer_list<int64_t> dims) {
 void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha,
               const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB,
               double beta, const double* C, int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD,
-              int64_t batch = 1, int tag = 0);
+              int64_t batch = 1, int tag = 0, double* work = nullptr, int64_t work_elems = 0);
+// workspace (doubles) that gemm_f64 would use to split K for this shape (0: no split)
+int64_t gemm_splitk_elems(int64_t M, int64_t N, int64_t K, int64_t batch);
 // tag selects a distinct kernel symbol for profiling: 0 generic, 1 PP ladder, 2 (T) W build
 
 // out[perm] = alpha * in + beta * out: out dim d has size odim[d] and reads input with stride istr[d].
@@ -120,8 +122,11 @@ struct TriplesDesc {
   const double* G[6];
   int64_t sa[6], sb[6], sc[6], sk[6];
 };
-void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TriplesDesc& d,
-                    const double* t1p, const double* vvoo, const double* eo, const double* ev,
-                    double* partials, int64_t* nblocks);
+// orbits: device list of cube-block triples (A>=B>=C), 3 ints each (triples_orbits)
+int triples_orbit_count(int v);
+void triples_orbits(int v, std::vector<int>& out);
+void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TriplesDesc& d, const int* orbits,
+                    const double* t1p, const double* vvoo, const double* eo, const double* ev, double* partials,
+                    int64_t* nblocks);
 
 }  // namespace fh

@@ -322,71 +322,136 @@ void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, 
 }
 
 // ------------------------------------------------------------------ (T) energy epilogue (ijk.jl:116-136)
-// One thread per virtual pair a>=b (decoded from the linear pair index), loop over c<=b.
-// W at the six permutations of (a,b,c) is gathered from the six GEMM class buffers; V adds
-// the T1 x (ia|jb) terms; the energy uses the reference's restricted-sum weights.
-__device__ __forceinline__ double tW(const TriplesDesc& d, int64_t a, int64_t b, int64_t c, int64_t k) {
-  double w = 0.0;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) w += d.G[q][a * d.sa[q] + b * d.sb[q] + c * d.sc[q] + k * d.sk[q]];
-  return w;
-}
+// For one occupied triple the reference's restricted sum over a>=b>=c with weights
+// 1/(1+d_ab+d_bc) equals the full sum over ordered (a,b,c) of
+//     W_abc * (4V_abc + V_bca + V_cab - 2V_acb - 2V_bac - 2V_cba) / D_abc
+// (the summand Ef of ijk.jl:132 is symmetric in (a,b,c); a=b=c terms vanish), times
+// (2 - d_ij - d_jk).  The virtual space is cut into 8x8x8 cubes; one workgroup takes an
+// orbit {A,B,C} (A>=B>=C) of cube blocks, assembles W on the (up to) six permuted cubes in
+// LDS from the six GEMM class buffers -- each class read along its unit-stride axis, so
+// every 8-element run is one 64-byte segment -- builds V = W + T1 x (ia|jb) terms, and sums
+// the orbit.  Each element of each class buffer is read exactly once per triple.
+static constexpr int CT = 8, CT3 = CT * CT * CT;
 
-__global__ __launch_bounds__(RB) void k_triples_energy(int o, int v, int i, int j, int k0, TriplesDesc d,
-                                                       const double* __restrict__ t1p, const double* __restrict__ vvoo,
-                                                       const double* __restrict__ eo, const double* __restrict__ ev,
-                                                       double* __restrict__ partials) {
-  const int64_t npair = (int64_t)v * (v + 1) / 2;
-  const int kk = blockIdx.y;
-  const int k = k0 + kk;
-  const double dij = (i == j) ? 1.0 : 0.0, djk = (j == k) ? 1.0 : 0.0;
-  const double wocc = 2.0 - dij - djk;
-  const double Dijk = eo[i] + eo[j] + eo[k];
-  const int64_t vv = (int64_t)v * v;
-  const double* vjk = vvoo + vv * (j + (int64_t)o * k);  // vvoo[:,:,j,k]
-  const double* vik = vvoo + vv * (i + (int64_t)o * k);
-  const double* vij = vvoo + vv * (i + (int64_t)o * j);
+__global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int j, int k0, TriplesDesc d,
+                                                        const int* __restrict__ orbits, int norb,
+                                                        const double* __restrict__ t1p, const double* __restrict__ vvoo,
+                                                        const double* __restrict__ eo, const double* __restrict__ ev,
+                                                        double* __restrict__ partials) {
+  __shared__ double Wc[6][CT3];
+  __shared__ double Vc[6][CT3];
+  __shared__ int slot_of[6];
+  const int tid = threadIdx.x;
+  const int kk = blockIdx.y, k = k0 + kk;
+  const int ob = blockIdx.x;
+  const double wocc = 2.0 - (i == j ? 1.0 : 0.0) - (j == k ? 1.0 : 0.0);
+  const int A = orbits[3 * ob], B = orbits[3 * ob + 1], C = orbits[3 * ob + 2];
+  // the six orderings of the orbit's blocks: (abc) (acb) (bac) (bca) (cab) (cba)
+  const int pb[6][3] = {{A, B, C}, {A, C, B}, {B, A, C}, {B, C, A}, {C, A, B}, {C, B, A}};
+  if (tid < 6) {
+    int s = tid;
+    for (int q = 0; q < tid; ++q)
+      if (pb[q][0] == pb[tid][0] && pb[q][1] == pb[tid][1] && pb[q][2] == pb[tid][2]) { s = q; break; }
+    slot_of[tid] = s;
+  }
   double acc = 0.0;
-  const int64_t p = (int64_t)blockIdx.x * RB + threadIdx.x;
-  if (wocc != 0.0 && p < npair) {
-    int64_t a = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
-    while (a * (a + 1) / 2 > p) --a;
-    while ((a + 1) * (a + 2) / 2 <= p) ++a;
-    const int64_t b = p - a * (a + 1) / 2;
-    const double ta = t1p[a + (int64_t)v * i], tb = t1p[b + (int64_t)v * j];
-    const double Dab = Dijk - ev[a] - ev[b];
-    const double dab = (a == b) ? 1.0 : 0.0;
-    for (int64_t c = 0; c <= b; ++c) {
-      const double Wabc = tW(d, a, b, c, kk), Wacb = tW(d, a, c, b, kk), Wbac = tW(d, b, a, c, kk);
-      const double Wbca = tW(d, b, c, a, kk), Wcab = tW(d, c, a, b, kk), Wcba = tW(d, c, b, a, kk);
-      // V[x,y,z] = W[x,y,z] + T1p[x,i] vvoo[y,z,j,k] + vvoo[x,z,i,k] T1p[y,j] + vvoo[x,y,i,j] T1p[z,k]
-      auto Vf = [&](int64_t x, int64_t y, int64_t z, double w) {
-        return w + t1p[x + (int64_t)v * i] * vjk[y + v * z] + vik[x + v * z] * t1p[y + (int64_t)v * j] +
-               vij[x + v * y] * t1p[z + (int64_t)v * k];
-      };
-      const double Vabc = Vf(a, b, c, Wabc), Vacb = Vf(a, c, b, Wacb), Vbac = Vf(b, a, c, Wbac);
-      const double Vbca = Vf(b, c, a, Wbca), Vcab = Vf(c, a, b, Wcab), Vcba = Vf(c, b, a, Wcba);
-      const double X = Wabc * Vabc + Wacb * Vacb + Wbac * Vbac + Wbca * Vbca + Wcab * Vcab + Wcba * Vcba;
-      const double Y = Vabc + Vbca + Vcab;
-      const double Z = Vacb + Vbac + Vcba;
-      const double Ef = (Y - 2.0 * Z) * (Wabc + Wbca + Wcab) + (Z - 2.0 * Y) * (Wacb + Wbac + Wcba) + 3.0 * X;
-      const double Dd = Dab - ev[c];
-      const double dbc = (b == c) ? 1.0 : 0.0;
-      acc += Ef * wocc / (Dd * (1.0 + dab + dbc));
+  if (wocc != 0.0) {
+    for (int p = 0; p < 6; ++p)
+      for (int e = tid; e < CT3; e += 256) Wc[p][e] = 0.0;
+    __syncthreads();
+    const int64_t vv = (int64_t)v * v;
+    // assemble W on every distinct cube, class by class
+    for (int p = 0; p < 6; ++p) {
+      if (slot_of[p] != p) continue;
+      const int P0 = pb[p][0] * CT, Q0 = pb[p][1] * CT, R0 = pb[p][2] * CT;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const int u = q >> 1;  // unit-stride axis of class q: 0 -> a, 1 -> b, 2 -> c
+        const double* G = d.G[q];
+        const int64_t sa = d.sa[q], sb = d.sb[q], sc = d.sc[q], skk = d.sk[q] * kk;
+        for (int e = tid; e < CT3; e += 256) {
+          const int x = e & 7, y = (e >> 3) & 7, z = e >> 6;
+          int la, lb, lc;
+          if (u == 0) { la = x; lb = y; lc = z; }
+          else if (u == 1) { lb = x; la = y; lc = z; }
+          else { lc = x; la = y; lb = z; }
+          const int a = P0 + la, b = Q0 + lb, c = R0 + lc;
+          if (a < v && b < v && c < v) Wc[p][la + CT * lb + CT * CT * lc] += G[a * sa + b * sb + c * sc + skk];
+        }
+        __syncthreads();
+      }
     }
-    (void)ta; (void)tb;
+    // V = W + T1p[x,i] vvoo[y,z,j,k] + vvoo[x,z,i,k] T1p[y,j] + vvoo[x,y,i,j] T1p[z,k]   (ijk.jl:116)
+    const double* vjk = vvoo + vv * (j + (int64_t)o * k);
+    const double* vik = vvoo + vv * (i + (int64_t)o * k);
+    const double* vij = vvoo + vv * (i + (int64_t)o * j);
+    for (int p = 0; p < 6; ++p) {
+      if (slot_of[p] != p) continue;
+      const int P0 = pb[p][0] * CT, Q0 = pb[p][1] * CT, R0 = pb[p][2] * CT;
+      for (int e = tid; e < CT3; e += 256) {
+        const int a = P0 + (e & 7), b = Q0 + ((e >> 3) & 7), c = R0 + (e >> 6);
+        double val = 0.0;
+        if (a < v && b < v && c < v)
+          val = Wc[p][e] + t1p[a + (int64_t)v * i] * vjk[b + (int64_t)v * c] + vik[a + (int64_t)v * c] * t1p[b + (int64_t)v * j] +
+                vij[a + (int64_t)v * b] * t1p[c + (int64_t)v * k];
+        Vc[p][e] = val;
+      }
+    }
+    __syncthreads();
+    const double Dijk = eo[i] + eo[j] + eo[k];
+    for (int p = 0; p < 6; ++p) {
+      if (slot_of[p] != p) continue;
+      const int X = pb[p][0], Y = pb[p][1], Z = pb[p][2];
+      // slots holding the cubes (Y,Z,X) (Z,X,Y) (X,Z,Y) (Y,X,Z) (Z,Y,X)
+      auto find = [&](int x0, int y0, int z0) {
+        int r = 0;
+        for (int q = 0; q < 6; ++q)
+          if (pb[q][0] == x0 && pb[q][1] == y0 && pb[q][2] == z0) { r = slot_of[q]; break; }
+        return r;
+      };
+      const int s_bca = find(Y, Z, X), s_cab = find(Z, X, Y), s_acb = find(X, Z, Y), s_bac = find(Y, X, Z),
+                s_cba = find(Z, Y, X);
+      for (int e = tid; e < CT3; e += 256) {
+        const int la = e & 7, lb = (e >> 3) & 7, lc = e >> 6;
+        const int a = X * CT + la, b = Y * CT + lb, c = Z * CT + lc;
+        if (a < v && b < v && c < v) {
+          const double Vabc = Vc[p][e];
+          const double Vbca = Vc[s_bca][lb + CT * lc + CT * CT * la];
+          const double Vcab = Vc[s_cab][lc + CT * la + CT * CT * lb];
+          const double Vacb = Vc[s_acb][la + CT * lc + CT * CT * lb];
+          const double Vbac = Vc[s_bac][lb + CT * la + CT * CT * lc];
+          const double Vcba = Vc[s_cba][lc + CT * lb + CT * CT * la];
+          const double Zs = 4.0 * Vabc + Vbca + Vcab - 2.0 * (Vacb + Vbac + Vcba);
+          acc += Wc[p][e] * Zs / (Dijk - ev[a] - ev[b] - ev[c]);
+        }
+      }
+    }
+    acc *= wocc;
   }
   const double t = block_sum(acc);
-  if (threadIdx.x == 0) partials[blockIdx.x + (int64_t)gridDim.x * kk] = t;
+  if (tid == 0) partials[blockIdx.x + (int64_t)gridDim.x * kk] = t;
 }
 
-void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TriplesDesc& d, const double* t1p,
-                    const double* vvoo, const double* eo, const double* ev, double* partials, int64_t* nblocks) {
-  const int64_t npair = (int64_t)v * (v + 1) / 2;
-  const int64_t gx = (npair + RB - 1) / RB;
-  *nblocks = gx * nk;
-  hipLaunchKernelGGL(k_triples_energy, dim3((unsigned)gx, (unsigned)nk), dim3(RB), 0, s, o, v, i, j, k0, d, t1p, vvoo,
-                     eo, ev, partials);
+int triples_orbit_count(int v) {
+  const int nb = (v + CT - 1) / CT;
+  return nb * (nb + 1) * (nb + 2) / 6;
+}
+
+void triples_orbits(int v, std::vector<int>& out) {
+  const int nb = (v + CT - 1) / CT;
+  out.clear();
+  for (int a = 0; a < nb; ++a)
+    for (int b = 0; b <= a; ++b)
+      for (int c = 0; c <= b; ++c) { out.push_back(a); out.push_back(b); out.push_back(c); }
+}
+
+void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TriplesDesc& d, const int* orbits,
+                    const double* t1p, const double* vvoo, const double* eo, const double* ev, double* partials,
+                    int64_t* nblocks) {
+  const int norb = triples_orbit_count(v);
+  *nblocks = (int64_t)norb * nk;
+  hipLaunchKernelGGL(k_triples_energy, dim3((unsigned)norb, (unsigned)nk), dim3(256), 0, s, o, v, i, j, k0, d, orbits,
+                     norb, t1p, vvoo, eo, ev, partials);
   FH_HIP(hipGetLastError());
 }
 
