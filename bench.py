@@ -35,13 +35,18 @@ def phase_diff(after, before):
 
 
 def cpu_baseline(o, v, Q, seed, niter, budget_s):
-    """Time the CPU oracle (numpy restatement of Fermi's algorithm) on a bounded sample of the
-    same workload and scale to a time-to-solution estimate:
-      t = t_integrals + t_guess + niter * t_iteration + t_(T)
-    t_(T) is measured on a uniform-stride sample of occupied triples (the per-triple cost of
-    ijk.jl is uniform); t_iteration is one full oracle update_amp + energy when the shape
-    allows, otherwise it is measured on the (T)-free part at the same shape by timing a
-    uniform-stride subset of the DF ladder's virtual loop plus one term of each cost class."""
+    """CPU oracle (numpy restatement of Fermi's RCCSDa / ijk.jl) on a bounded sample of the same
+    workload, scaled to a time-to-solution estimate
+        t = t_integrals + niter * (t_ladder + t_rest) + t_(T):
+      * t_integrals: the 4-index blocks other than VVVV, timed in full;
+      * t_(T): ijk.jl restatement on a uniform-stride sample of occupied triples, scaled by the
+        triple count (the per-triple cost is uniform);
+      * t_ladder: the DF ladder's virtual loop (RCCSDHelper.jl:214-219) on a uniform subset of
+        virtuals, scaled by v (uniform cost per virtual);
+      * t_rest: all other terms of one iteration (T1 numerator, non-ladder T2 terms, energy),
+        timed in full at the (H2O)4 shape and scaled by the o^3 v^3 ratio of the two shapes
+        (their dominant term class), unless this workload is that small itself.
+    Returns the cpu_baseline object of the benchmark line."""
     import numpy as np
     from fermihip import synth
     from oracle import rccsd as R
@@ -53,54 +58,48 @@ def cpu_baseline(o, v, Q, seed, niter, budget_s):
     T1, T2 = R.mp2_guess(m, eo, ev)
     trip = TT.triple_list(o)
     ntr = len(trip)
-    # (T) sample: about a third of the budget
     t0 = time.perf_counter()
-    TT.rccsdpt_ijk(T1, T2, m, eo, ev, triples=[trip[0]])
+    TT.rccsdpt_ijk(T1, T2, m, eo, ev, triples=[trip[len(trip) // 2]])
     per = max(time.perf_counter() - t0, 1e-6)
-    ns = int(max(2, min(ntr, (budget_s / 3.0) / per)))
-    stride = max(1, ntr // ns)
-    sample = trip[::stride][:ns]
+    ns = int(max(2, min(ntr, (budget_s * 0.5) / per)))
+    sample = trip[:: max(1, ntr // ns)][:ns]
     t0 = time.perf_counter()
     TT.rccsdpt_ijk(T1, T2, m, eo, ev, triples=sample)
     t_T = (time.perf_counter() - t0) / len(sample) * ntr
-    # CCSD iteration
+    # ladder: uniform subset of the virtual loop
+    na = max(1, min(v, v // 16))
+    avals = list(range(0, v, max(1, v // na)))[:na]
     t0 = time.perf_counter()
-    n1 = R.cc_update_T1(T1, T2, m)
-    t_t1 = time.perf_counter() - t0
-    t_t2 = None
-    if (o ** 3) * (v ** 3) < 2e11:
-        t0 = time.perf_counter()
-        R.cc_update_T2(T1, T2, m, df=True)
-        t_t2 = time.perf_counter() - t0
-        how = "full oracle update_amp"
+    for a in avals:
+        cdb = np.tensordot(Bvv[:, :, a], Bvv, axes=(0, 0))                # cdb[c,d,b]   (:216)
+        np.tensordot(T2, cdb, axes=([2, 3], [0, 1]))                       # tau[i,j,c,d] cdb[c,d,b] (:217)
+    t_ladder = (time.perf_counter() - t0) / len(avals) * v
+    # rest of an iteration at the (H2O)4 shape (or at this shape when it is that small)
+    so, sv, sq = synth.SHAPES["water4"]
+    if o * v <= so * sv:
+        so, sv, sq, ms, e1, e2 = o, v, Q, m, T1, T2
     else:
-        # ladder: uniform-stride subset of the virtual loop (RCCSDHelper.jl:214) scaled by v
-        tau = T2
-        na = max(1, v // 16)
-        t0 = time.perf_counter()
-        for a in range(0, v, v // na)[:na]:
-            cdb = np.einsum("Qc,Qdb->cdb", Bvv[:, :, a], Bvv, optimize=True)
-            np.einsum("ijcd,cdb->ijb", tau, cdb, optimize=True)
-        t_lad = (time.perf_counter() - t0) / na * v
-        # remaining terms: the o^3 v^3 ring class dominates; time one term of that class and
-        # count the class members of RCCSDHelper.jl:141-188 (33 terms with an o^3v^3 pairwise step)
-        t0 = time.perf_counter()
-        np.einsum("ikac,kcld->iald", T2, m["OVOV"], optimize=True)
-        t_ring = time.perf_counter() - t0
-        t_t2 = t_lad + 33 * t_ring
-        how = f"ladder sampled over {na}/{v} virtuals + 33 x one o^3v^3 term"
+        a_, b_, c_, d_, f_ = synth.molecular_like(so, sv, sq, seed)
+        ms = R.build_moints(c_, d_, f_, with_vvvv=False)
+        e1, e2 = R.mp2_guess(ms, a_, b_)
+    ms_l = dict(ms)
+    ms_l["BVV"] = np.zeros((1, sv, sv))                                     # ladder timed separately
     t0 = time.perf_counter()
-    R.cc_update_energy(T1, T2, m)
-    t_e = time.perf_counter() - t0
-    t_iter = t_t1 + t_t2 + t_e
-    total = t_int + niter * t_iter + t_T
+    R.cc_update_T1(e1, e2, ms_l)
+    R.cc_update_T2(e1, e2, ms_l, df=True)
+    R.cc_update_energy(e1, e2, ms_l)
+    t_rest_small = time.perf_counter() - t0
+    scale = (o ** 3 * v ** 3) / float(so ** 3 * sv ** 3)
+    t_rest = t_rest_small * scale
+    total = t_int + niter * (t_ladder + t_rest) + t_T
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": total, "unit": "s", "cores": threads, "kind": "port",
-        "sample": (f"numpy oracle (restatement of RCCSDa/RCCSDHelper/ijk.jl), same synthetic workload: "
-                   f"integrals timed in full; (T) on {len(sample)} of {ntr} triples (uniform stride) x {ntr}/{len(sample)}; "
-                   f"one CCSD iteration ({how}) x {niter} iterations of the GPU run"),
-        "parts_s": {"integrals": t_int, "iteration": t_iter, "triples": t_T},
+        "sample": (f"numpy oracle of RCCSDa/RCCSDHelper.jl/ijk.jl on the same synthetic workload: integrals in full; "
+                   f"(T) on {len(sample)} of {ntr} triples (uniform stride) x {ntr}/{len(sample)}; DF ladder on "
+                   f"{len(avals)} of {v} virtuals x {v}/{len(avals)}; other CCSD terms timed at o={so},v={sv} x "
+                   f"{scale:.1f} (o^3v^3 ratio); x {niter} iterations of the GPU run"),
+        "parts_s": {"integrals": t_int, "ladder_per_iter": t_ladder, "rest_per_iter": t_rest, "triples": t_T},
     }
 
 
@@ -114,6 +113,9 @@ def main():
     ap.add_argument("--diis-prec", default="single", choices=["single", "double"])
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--phase", default="full", choices=["full", "triples", "copy"],
+                    help="full: the benchmark step; triples: integrals + guess + (T) only (profiling runs); "
+                         "copy: one plain 1 GiB device copy (calibration of the HBM byte counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -121,6 +123,17 @@ def main():
         print("bench.py: this DF-RCCSD(T) engine runs on one GPU; multi-GPU runs are not part of this build "
               "(see DESIGN.md for how the path would shard).", file=sys.stderr)
         sys.exit(2)
+
+    if args.phase == "copy":
+        import torch
+        n = 1 << 27                                  # 2^27 doubles = 1 GiB
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"phase": "copy", "bytes_read": n * 8, "bytes_written": n * 8}), flush=True)
+        return
 
     import fermihip
     from fermihip import synth
@@ -137,10 +150,23 @@ def main():
         t1 = time.perf_counter()
         ctx.prepare()
         ctx.guess(copy_out=False)
-        r, _, _ = ctx.solve(opts, copy_out=False)
+        if args.phase == "triples":
+            r = None
+        else:
+            r, _, _ = ctx.solve(opts, copy_out=False)
         Et = ctx.triples()
         t2 = time.perf_counter()
         return t2 - t1, t1 - t0, r, Et
+
+    if args.phase == "triples":
+        for k in range(max(1, args.steps)):
+            dt, up, _, Et = step()
+            log(f"bench: (T)-only step {k} {dt:.3f} s Et(MP2 amplitudes)={Et:.12e}")
+        st = ctx.stats()["phases"]
+        print(json.dumps({"phase": "triples", "config": args.config, "seconds": dt, "Et": Et,
+                          "triples_gemm": st.get("triples_gemm"), "triples_energy": st.get("triples_energy")}), flush=True)
+        ctx.close()
+        return
 
     for w in range(args.warmup):
         dt, up, r, Et = step()
@@ -164,14 +190,25 @@ def main():
                 "calls": p["calls"] // max(1, args.steps), "flops": p["flops"] / max(1, args.steps)}
 
     kern = {n: rate(n) for n in ph}
-    dom = max(("ladder", "triples_gemm"), key=lambda n: (kern[n]["ms"] if kern[n] else -1))
+    # dominant MFMA kernel of the step: the PP ladder GEMM or the (T) W-build GEMM (distinct
+    # kernel symbols via the tag template argument, so rocprofv3 lists them separately)
+    launches_per_call = {"ladder": 1, "triples_gemm": 3}
+    names = {"ladder": "k_gemm_f64<64, 64, 16, false, false, 4, 1> (PP ladder, tag 1)",
+             "triples_gemm": "k_gemm_f64<64, 64, 8, false, false, 4, 2> ((T) W build, tag 2)"}
+    dom = max(("ladder", "triples_gemm"), key=lambda n: (kern[n]["ms"] if kern.get(n) else -1))
     kd = kern[dom]
-    roof = {"kernel": {"ladder": "k_gemm_f64<128,128,16,false,false,1,1> (PP ladder)",
-                       "triples_gemm": "k_gemm_f64<128,128,16,false,false,1,2> ((T) W build)"}[dom],
-            "bound": "mfma", "achieved": kd["tflops"], "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": kd["tflops"] / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
-            "flops_per_launch": kd["flops"] / max(1, kd["calls"]),
-            "avg_launch_us": kd["ms"] * 1e3 / max(1, kd["calls"])}
+    nl = kd["calls"] * launches_per_call[dom]
+    roof = {"kernel": names[dom], "bound": "mfma", "achieved": kd["tflops"], "peak": FP64_MFMA_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": kd["tflops"] / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
+            "flops_per_launch": kd["flops"] / max(1, nl), "avg_launch_us": kd["ms"] * 1e3 / max(1, nl),
+            "launches": nl}
+    other = "ladder" if dom == "triples_gemm" else "triples_gemm"
+    roof_other = None
+    if kern.get(other):
+        ko = kern[other]
+        no = ko["calls"] * launches_per_call[other]
+        roof_other = {"kernel": names[other], "achieved": ko["tflops"], "frac": ko["tflops"] / FP64_MFMA_PEAK_TFLOPS,
+                      "avg_launch_us": ko["ms"] * 1e3 / max(1, no), "launches": no}
     out = {
         "metric": "DF-RCCSD(T) time-to-solution (s) + MFMA TFLOP/s at <=1 uEh vs FP64 GPU path",
         "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -184,6 +221,7 @@ def main():
         "energies": {"ccsd_correlation": r.correlation, "triples": Et, "niter": r.niter, "converged": bool(r.converged)},
         "pcie_inclusive_s": tts + sum(uploads) / len(uploads),
         "roofline": roof,
+        "roofline_second": roof_other,
         "kernels": {k: vv for k, vv in kern.items() if vv},
     }
     if args.cpu_baseline == "auto":

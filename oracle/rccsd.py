@@ -8,9 +8,8 @@ never calls it.
 Every function restates the reference term by term, with the same index names and
 array index order as the Julia source (numpy arrays here are indexed exactly like
 the Julia arrays, e.g. ``T2[i,j,a,b]``, ``Vovov[i,a,j,b] = (ia|jb)``).  Julia's
-``@tensor``/``@tensoropt`` pairwise contractions are restated with
-``numpy.einsum(..., optimize=True)`` (pairwise contractions over BLAS), which is
-the same evaluation strategy TensorOperations.jl uses.
+``@tensor``/``@tensoropt`` pairwise contractions are restated with ``oracle.contract.ein``
+(pairwise numpy.tensordot over BLAS), the evaluation strategy TensorOperations.jl uses.
 
 Pins (see ``tests/test_oracle_*.py``):
   * ``cc_update_energy``, ``cc_update_T1``, ``cc_update_T2``, ``update_amp``: against a
@@ -24,6 +23,8 @@ Pins (see ``tests/test_oracle_*.py``):
 """
 import numpy as np
 
+from .contract import ein as _pairwise
+
 __all__ = [
     "build_moints", "mp2_guess", "cc_update_energy", "cc_update_T1", "cc_update_T2",
     "df_ladder", "conv_ladder", "update_amp", "DIISManager", "rccsd_solve", "Options",
@@ -31,7 +32,7 @@ __all__ = [
 
 
 def _ein(sub, *ops):
-    return np.einsum(sub, *ops, optimize=True)
+    return _pairwise(sub, *ops)
 
 
 # Defaults of the CC keys read on the hot path: src/Core/Options.jl:62-102.

@@ -16,6 +16,8 @@ under relabelling orbitals within o and within v.
 """
 import numpy as np
 
+from .contract import ein
+
 __all__ = ["rccsdpt_ijk", "rccsdpt_abc", "rccsdpt_ijk2", "triple_list"]
 
 
@@ -62,7 +64,7 @@ def rccsdpt_ijk(T1, T2, moints, fo, fv, triples=None):
     Vvvoo = _perm(moints["OVOV"], (2, 4, 1, 3))   # Vvvoo[a,b,i,j]=(ia|jb)    :32
     v, o = T1p.shape
     idx = _abc_lower(v)
-    e = lambda s, x, y: np.einsum(s, x, y, optimize=True)
+    e = ein
     Et = 0.0
     for (i, j, k) in (triple_list(o) if triples is None else triples):
         Dijk = fo[i] + fo[j] + fo[k]
@@ -96,7 +98,7 @@ def rccsdpt_abc(T1, T2, moints, fo, fv, faithful=True):
     ii, jj, kk = ii[m], jj[m], kk[m]
     dij = (ii == jj).astype(np.float64)
     djk = (jj == kk).astype(np.float64)
-    e = lambda s, x, y: np.einsum(s, x, y, optimize=True)
+    e = ein
     Et = 0.0
     for a in range(v):
         for b in range(a + 1):
