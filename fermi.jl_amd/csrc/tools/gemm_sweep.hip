@@ -11,7 +11,7 @@ double run(int64_t M, int64_t N, int64_t K, const double* A, const double* B, do
   const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   hipEvent_t a, b; hipEventCreate(&a); hipEventCreate(&b);
   auto go = [&]() { hipLaunchKernelGGL((k_gemm_f64<BM, BN, BK, false, false, OCC, 9>), dim3(tm * tn, 1), dim3(256), 0, 0,
-                     M, N, K, 1.0, A, M, 0, B, K, 0, 0.0, nullptr, M, 0, D, M, 0, (int)tm, (int)tn, (int64_t)0); };
+                     M, N, K, 1.0, A, M, 0, B, K, 0, 0.0, nullptr, M, 0, D, M, 0, (int)tm, (int)tn, (int64_t)0, M > N ? 1 : 0); };
   go(); hipDeviceSynchronize();
   hipEventRecord(a);
   for (int r = 0; r < reps; ++r) go();
