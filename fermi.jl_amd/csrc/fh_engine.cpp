@@ -720,26 +720,30 @@ static double triples(Ctx& c) {
   const double o_ = (double)o, v_ = (double)v;
   const double ntrip = o_ * (o_ + 1) * (o_ + 2) / 6.0;
   Timer tall(c, "triples", ntrip * 12.0 * (v_ * v_ * v_ * v_ + o_ * v_ * v_ * v_));
-  // BVq (Bcat_{p,q} stored [d',z,q,p]) and BVp (stored [d',z,p,q]) share one allocation so that
-  // the two classes that use them with the same offsets run as one strided-batch GEMM
-  const int64_t bvsz = nd * v * o * o;
-  Buf AV = dalloc(c, vv * o * nd), BVqp = dalloc(c, 2 * bvsz);
+  // AV[(x,y),n,d'] and the row-swapped copy AVs[(y,x),n,d'] share one allocation;
+  // BVq (Bcat_{p,q} stored [d',z,q,p]) and BVp (stored [d',z,p,q]) likewise
+  const int64_t avsz = vv * o * nd, bvsz = nd * v * o * o;
+  Buf AVall = dalloc(c, 2 * avsz), BVqp = dalloc(c, 2 * bvsz);
+  double* AV = AVall.d();
+  double* AVs = AVall.d() + avsz;
   double* BVq = BVqp.d();
   double* BVp = BVqp.d() + bvsz;
   Buf T1p = dalloc(c, v * o), vvoo = dalloc(c, vv * o * o);
   {
     auto g = [](int64_t n) { int64_t b = (n + 255) / 256; return (unsigned)std::min<int64_t>(b, 65536 * 4); };
-    hipLaunchKernelGGL(k_build_av, dim3(g(vv * o * nd)), dim3(256), 0, c.s, (int)o, (int)v, c.OVVV.d(), c.T2.d(), AV.d());
+    hipLaunchKernelGGL(k_build_av, dim3(g(avsz)), dim3(256), 0, c.s, (int)o, (int)v, c.OVVV.d(), c.T2.d(), AV);
     hipLaunchKernelGGL(k_build_bv, dim3(g(bvsz)), dim3(256), 0, c.s, (int)o, (int)v, false, c.T2.d(), c.OOOV.d(), BVp);
     hipLaunchKernelGGL(k_build_bv, dim3(g(bvsz)), dim3(256), 0, c.s, (int)o, (int)v, true, c.T2.d(), c.OOOV.d(), BVq);
     FH_HIP(hipGetLastError());
+    perm(c, tv(AV, {v, v, o, nd}), "xynd", tv(AVs, {v, v, o, nd}), "yxnd", 1.0, 0.0);
     perm(c, tv(c.T1.d(), {o, v}), "ia", tv(T1p.d(), {v, o}), "ai", 1.0, 0.0);
     perm(c, tv(c.OVOV.d(), {o, v, o, v}), "iajb", tv(vvoo.d(), {v, v, o, o}), "abij", 1.0, 0.0);
   }
-  // k-chunk: bound the six class buffers to ~16 GB
-  int64_t KC = std::max<int64_t>(1, std::min<int64_t>(o, (int64_t)(16e9 / (6.0 * 8.0 * (double)vv * v))));
-  const int64_t gsz = vv * v * KC;
-  Buf Gall = dalloc(c, 6 * gsz);
+  // k-chunk: W for all k of a chunk, bounded to ~16 GB
+  const int64_t v3 = vv * v;
+  int64_t KC = std::max<int64_t>(1, std::min<int64_t>(o, (int64_t)(16e9 / (8.0 * (double)v3))));
+  Buf Wbuf = dalloc(c, v3 * KC);
+  double* W = Wbuf.d();
   std::vector<int> orb;
   triples_orbits((int)v, orb);
   const int64_t norb = (int64_t)orb.size() / 3;
@@ -750,38 +754,42 @@ static double triples(Ctx& c) {
   Buf acc = dalloc(c, 1);
   fill_f64(c.s, 1, 0.0, acc.d());
   const int64_t bstride = nd * v;  // one (p,q) column block of BVp/BVq
-  double* G = Gall.d();
+  // output maps into W[a + v*b + v^2*c + v^3*kk]
+  OutMap m_cab, m_cba, m_acb, m_bca;
+  m_cab.rmod = v; m_cab.rs0 = 1; m_cab.rs1 = vv; m_cab.cmod = v; m_cab.cs0 = v; m_cab.cs1 = v3;   // rows (a,c), cols (b,kk)
+  m_cba.rmod = v; m_cba.rs0 = v; m_cba.rs1 = vv; m_cba.cmod = v; m_cba.cs0 = 1; m_cba.cs1 = v3;   // rows (b,c), cols (a,kk)
+  m_acb.rmod = v; m_acb.rs0 = 1; m_acb.rs1 = vv; m_acb.cmod = 0; m_acb.cs0 = v;                   // rows (a,c), col b
+  m_bca.rmod = v; m_bca.rs0 = v; m_bca.rs1 = vv; m_bca.cmod = 0; m_bca.cs0 = 1;                   // rows (b,c), col a
   for (int64_t i = 0; i < o; ++i)
     for (int64_t j = 0; j <= i; ++j) {
       const int64_t kend = (i == j) ? j : j + 1;  // i=j=k has zero weight (ijk.jl:133)
       for (int64_t k0 = 0; k0 < kend; k0 += KC) {
         const int64_t nk = std::min<int64_t>(KC, kend - k0);
         std::unique_ptr<Timer> tg(new Timer(c, "triples_gemm", 12.0 * (double)vv * (double)v * (double)nd * (double)nk));
-        // class slots: 0 abc, 1 cba (A = AV_j) | 2 bac, 3 cab (A = AV_i) | 4 acb, 5 bca (A = AV_k rows)
-        // N-extended: G[(x,y),(z,k)] = AV_n (v^2 x nd) * [Bcat columns (z,k)] (nd x v*nk); the two
-        // classes of a pair read BVq / BVp at the same offset -> batch of 2 with sB = bvsz
-        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AV.d() + vv * j, vv * o, 0, BVq + bstride * (k0 + o * i), nd,
-                 bvsz, 0.0, nullptr, vv, 0, G, vv, gsz, 2, 2);          // abc: B_{i,k}   cba: B_{k,i}
-        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AV.d() + vv * i, vv * o, 0, BVq + bstride * (k0 + o * j), nd,
-                 bvsz, 0.0, nullptr, vv, 0, G + 2 * gsz, vv, gsz, 2, 2); // bac: B_{j,k}   cab: B_{k,j}
-        // M-extended: G[(x,y,k), z] = AV rows (x,y,k) (v^2*nk x nd) * Bcat_{p,q} (nd x v)
-        gemm_f64(c.s, false, false, vv * nk, v, nd, 1.0, AV.d() + vv * k0, vv * o, 0, BVp + bstride * (i + o * j), nd,
-                 bstride * (o - 1) * (i - j), 0.0, nullptr, vv * nk, 0, G + 4 * gsz, vv * nk, gsz, 2, 2);  // acb: B_{i,j}  bca: B_{j,i}
+        // The six permutation classes of ijk2.jl:109-153, G = AV_n (rows (x,y)) * Bcat_{p,q}
+        // (cols z), accumulated into W[a,b,c,kk] (W[a,b,c] += G[class-permuted (a,b,c)]).
+        // abc: n=j, B_{i,k}, x=a y=b z=c
+        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AV + vv * j, vv * o, 0, BVq + bstride * (k0 + o * i), nd, 0,
+                 0.0, nullptr, vv, 0, W, vv, 0, 1, 2);
+        // bac: n=i, B_{j,k}, x=b y=a z=c  (row-swapped AV: rows (a,b))
+        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVs + vv * i, vv * o, 0, BVq + bstride * (k0 + o * j), nd, 0,
+                 1.0, W, vv, 0, W, vv, 0, 1, 2);
+        // cab: n=i, B_{k,j}, x=c y=a z=b  (rows (a,c), cols (b,kk))
+        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVs + vv * i, vv * o, 0, BVp + bstride * (k0 + o * j), nd, 0,
+                 1.0, W, vv, 0, W, vv, 0, 1, 2, nullptr, 0, &m_cab);
+        // cba: n=j, B_{k,i}, x=c y=b z=a  (rows (b,c), cols (a,kk))
+        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVs + vv * j, vv * o, 0, BVp + bstride * (k0 + o * i), nd, 0,
+                 1.0, W, vv, 0, W, vv, 0, 1, 2, nullptr, 0, &m_cba);
+        // acb: n=k, B_{i,j}, x=a y=c z=b  (batched over k: rows (a,c), col b)
+        gemm_f64(c.s, false, false, vv, v, nd, 1.0, AV + vv * k0, vv * o, vv, BVp + bstride * (i + o * j), nd, 0, 1.0, W,
+                 vv, v3, W, vv, v3, nk, 2, nullptr, 0, &m_acb);
+        // bca: n=k, B_{j,i}, x=b y=c z=a  (batched over k: rows (b,c), col a)
+        gemm_f64(c.s, false, false, vv, v, nd, 1.0, AV + vv * k0, vv * o, vv, BVp + bstride * (j + o * i), nd, 0, 1.0, W,
+                 vv, v3, W, vv, v3, nk, 2, nullptr, 0, &m_bca);
         tg.reset();
         Timer te(c, "triples_energy", 0.0);
-        TriplesDesc d;
-        for (int q = 0; q < 6; ++q) d.G[q] = G + q * gsz;
-        const int64_t v3 = vv * v, vk = vv * nk;
-        // classes in kernel order (unit-stride axis a, a, b, b, c, c):
-        //   abc x=a y=b z=c | acb x=a y=c k z=b | bac x=b y=a z=c | bca x=b y=c k z=a | cab x=c y=a z=b | cba x=c y=b z=a
-        d.G[0] = G;           d.sa[0] = 1;  d.sb[0] = v;  d.sc[0] = vv; d.sk[0] = v3;
-        d.G[1] = G + 4 * gsz; d.sa[1] = 1;  d.sc[1] = v;  d.sk[1] = vv; d.sb[1] = vk;
-        d.G[2] = G + 2 * gsz; d.sa[2] = v;  d.sb[2] = 1;  d.sc[2] = vv; d.sk[2] = v3;
-        d.G[3] = G + 5 * gsz; d.sb[3] = 1;  d.sc[3] = v;  d.sk[3] = vv; d.sa[3] = vk;
-        d.G[4] = G + 3 * gsz; d.sa[4] = v;  d.sb[4] = vv; d.sc[4] = 1;  d.sk[4] = v3;
-        d.G[5] = G + 1 * gsz; d.sa[5] = vv; d.sb[5] = v;  d.sc[5] = 1;  d.sk[5] = v3;
         int64_t nb = 0;
-        triples_energy(c.s, (int)o, (int)v, (int)i, (int)j, (int)k0, (int)nk, d, static_cast<const int*>(orbits.p),
+        triples_energy(c.s, (int)o, (int)v, (int)i, (int)j, (int)k0, (int)nk, W, static_cast<const int*>(orbits.p),
                        T1p.d(), vvoo.d(), c.eo.d(), c.ev.d(), part.d(), &nb);
         hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(256), 0, c.s, nb, part.d(), acc.d());
         FH_HIP(hipGetLastError());

@@ -22,7 +22,7 @@ template <int BM, int BN, int BK, bool TA, bool TB, int OCC, int TAG>
 __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
     int64_t M, int64_t N, int64_t K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
     const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
-    int64_t sC, double* D, int64_t ldd, int64_t sD, int tilesM, int tilesN, int64_t kchunk, int nfast) {
+    int64_t sC, double* D, int64_t ldd, int64_t sD, int tilesM, int tilesN, int64_t kchunk, int nfast, OutMap om) {
   constexpr int PADA = 16, PADB = 2;
   constexpr int WM = BM / 2, WN = BN / 2, MB = WM / 16, NB = WN / 16;
   constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
@@ -125,50 +125,57 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
     __syncthreads();
   }
 
-  // epilogue: lane holds rows (lane>>4)+4r, column lane&15 of each 16x16 block
+  // epilogue: lane holds rows (lane>>4)+4r, column lane&15 of each 16x16 block.  Output
+  // address: column-major with ldd, or (om.rmod > 0) the two-mode map
+  // row r -> (r % rmod)*rs0 + (r / rmod)*rs1, column c -> (c % cmod)*cs0 + (c / cmod)*cs1
+  // (cmod = 0: c*cs0), which lets a GEMM accumulate into a permuted 3-/4-index tensor.
 #pragma unroll
-  for (int i = 0; i < MB; ++i)
+  for (int j = 0; j < NB; ++j) {
+    const int64_t col = n0 + wn * WN + j * 16 + (lane & 15);
+    if (col >= N) continue;
+    int64_t coff;
+    if (om.rmod > 0) coff = om.cmod > 0 ? (col % om.cmod) * om.cs0 + (col / om.cmod) * om.cs1 : col * om.cs0;
+    else coff = col * ldd;
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      const int64_t col = n0 + wn * WN + j * 16 + (lane & 15);
-      if (col >= N) continue;
+    for (int i = 0; i < MB; ++i)
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int64_t row = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * rr;
         if (row < M) {
+          const int64_t roff = om.rmod > 0 ? (row % om.rmod) * om.rs0 + (row / om.rmod) * om.rs1 : row;
           double val = alpha * acc[i][j][rr];
-          if (beta != 0.0) val += beta * C[row + ldc * col];
-          D[row + ldd * col] = val;
+          if (beta != 0.0) val += beta * (om.rmod > 0 ? C[roff + coff] : C[row + ldc * col]);
+          D[roff + coff] = val;
         }
       }
-    }
+  }
 }
 
 template <int BM, int BN, int BK, bool TA, bool TB, int TAG>
 static void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                    int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
-                   int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk) {
+                   int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om) {
   const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   FH_CHECK(tm * tn < (int64_t)1 << 31 && batch < 65536, -3, "gemm grid too large");
   dim3 grid((unsigned)(tm * tn), (unsigned)batch);
   const int nfast = (M > N) ? 1 : 0;   // stream the larger of A (M x K) and B (K x N) once
   hipLaunchKernelGGL((k_gemm_f64<BM, BN, BK, TA, TB, (BM * BN > 8192 ? 1 : 4), TAG>), grid, dim3(256), 0, s, M, N, K, alpha, A,
-                     lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, nfast);
+                     lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, nfast, om);
   FH_HIP(hipGetLastError());
 }
 
 template <bool TA, bool TB, int TAG>
 static void pick(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                  int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
-                 int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk) {
+                 int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om) {
   // 64x64 tiles at up to four workgroups per CU measured fastest on every shape of this
   // engine (the latency of the next operand tile is hidden by the other workgroups);
   // BK = 8 when the per-tile K loop is short.
   const int64_t kl = kchunk > 0 ? kchunk : K;
   if (kl <= 512)
-    launch<64, 64, 8, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk);
+    launch<64, 64, 8, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk, om);
   else
-    launch<64, 64, 16, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk);
+    launch<64, 64, 16, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk, om);
 }
 
 // split-K sizing: few output tiles and a long K leave most CUs idle; cut K into slices
@@ -202,13 +209,15 @@ __global__ void k_splitk_reduce(int64_t M, int64_t N, int64_t S, const double* _
 void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
               int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C,
               int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int tag, double* work,
-              int64_t work_elems) {
+              int64_t work_elems, const OutMap* map) {
+  const OutMap om = map ? *map : OutMap{};
+  FH_CHECK(!map || (map->rmod > 0 && (beta == 0.0 || C == D)), -3, "gemm: mapped output needs rmod > 0 and C == D");
   if (M <= 0 || N <= 0 || batch <= 0) return;
   FH_CHECK(K >= 0, -3, "gemm: negative K");
   FH_CHECK(lda >= (ta ? (K > 0 ? K : 1) : M) && ldb >= (tb ? N : (K > 0 ? K : 1)) && ldd >= M, -3,
            "gemm: leading dimension too small");
   FH_CHECK(beta == 0.0 || (C != nullptr && ldc >= M), -3, "gemm: C missing");
-  const int64_t sl = splitk_slices(M, N, K, batch);
+  const int64_t sl = map ? 1 : splitk_slices(M, N, K, batch);
   const bool split = sl > 1 && work != nullptr && work_elems >= sl * M * N;
   const int64_t kchunk = split ? (K + sl - 1) / sl : 0;
   const int64_t nb = split ? (K + kchunk - 1) / kchunk : batch;
@@ -218,9 +227,9 @@ void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, 
   const int64_t lddd = split ? M : ldd, sDD = split ? M * N : sD;
 #define FH_PICK(TA_, TB_)                                                                                            \
   do {                                                                                                              \
-    if (tag == 1) pick<TA_, TB_, 1>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk); \
-    else if (tag == 2) pick<TA_, TB_, 2>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk); \
-    else pick<TA_, TB_, 0>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk);         \
+    if (tag == 1) pick<TA_, TB_, 1>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om); \
+    else if (tag == 2) pick<TA_, TB_, 2>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om); \
+    else pick<TA_, TB_, 0>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om);         \
   } while (0)
   if (ta) {
     if (tb) FH_PICK(true, true); else FH_PICK(true, false);

@@ -81,12 +81,19 @@ inline TView tv(double* p, std::initializer_list<int64_t> dims) {
 }
 
 // ---------------------------------------------------------------- kernel launchers
+// Output address map of a GEMM that accumulates into a permuted tensor (see k_gemm_f64):
+// row r -> (r % rmod)*rs0 + (r / rmod)*rs1, column c -> (c % cmod)*cs0 + (c / cmod)*cs1 (cmod = 0: c*cs0).
+struct OutMap {
+  int64_t rmod = 0, rs0 = 0, rs1 = 0, cmod = 0, cs0 = 0, cs1 = 0;
+};
+
 // D = alpha * op(A) * op(B) + beta * C  (column-major, FP64 MFMA); C may alias D.
 // Strided batches: A_i = A + i*sA etc.
 void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha,
               const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB,
               double beta, const double* C, int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD,
-              int64_t batch = 1, int tag = 0, double* work = nullptr, int64_t work_elems = 0);
+              int64_t batch = 1, int tag = 0, double* work = nullptr, int64_t work_elems = 0,
+              const OutMap* map = nullptr);
 // workspace (doubles) that gemm_f64 would use to split K for this shape (0: no split)
 int64_t gemm_splitk_elems(int64_t M, int64_t N, int64_t K, int64_t batch);
 // tag selects a distinct kernel symbol for profiling: 0 generic, 1 PP ladder, 2 (T) W build
@@ -116,16 +123,12 @@ void cast_f64_to_f32(hipStream_t s, int64_t n, const double* x, float* y);
 void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, const double* ovov,
                double* partials, double* result);
 
-// (T) energy for one (i,j) pair and k in [k0, k0+nk): sums into partials[blockIdx...].
-struct TriplesDesc {
-  // W[a,b,c] for occupied k = sum over the six classes of G[c][a*sa + b*sb + c*sc + k*sk]
-  const double* G[6];
-  int64_t sa[6], sb[6], sc[6], sk[6];
-};
+// (T) energy for one (i,j) pair and k in [k0, k0+nk): W[a + v*b + v^2*c + v^3*(k-k0)] holds the
+// W intermediate of ijk.jl:108-115; one FP64 partial per workgroup.
 // orbits: device list of cube-block triples (A>=B>=C), 3 ints each (triples_orbits)
 int triples_orbit_count(int v);
 void triples_orbits(int v, std::vector<int>& out);
-void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TriplesDesc& d, const int* orbits,
+void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const double* W, const int* orbits,
                     const double* t1p, const double* vvoo, const double* eo, const double* ev, double* partials,
                     int64_t* nblocks);
 

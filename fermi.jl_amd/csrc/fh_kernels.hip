@@ -326,15 +326,15 @@ void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, 
 // 1/(1+d_ab+d_bc) equals the full sum over ordered (a,b,c) of
 //     W_abc * (4V_abc + V_bca + V_cab - 2V_acb - 2V_bac - 2V_cba) / D_abc
 // (the summand Ef of ijk.jl:132 is symmetric in (a,b,c); a=b=c terms vanish), times
-// (2 - d_ij - d_jk).  The virtual space is cut into 8x8x8 cubes; one workgroup takes an
-// orbit {A,B,C} (A>=B>=C) of cube blocks, assembles W on the (up to) six permuted cubes in
-// LDS from the six GEMM class buffers -- each class read along its unit-stride axis, so
-// every 8-element run is one 64-byte segment -- builds V = W + T1 x (ia|jb) terms, and sums
-// the orbit.  Each element of each class buffer is read exactly once per triple.
+// (2 - d_ij - d_jk).  W[a,b,c] for the k-chunk lies in one buffer with a fastest
+// (W + a + v*b + v^2*c + v^3*kk).  The virtual space is cut into 8x8x8 cubes; one workgroup
+// takes an orbit {A,B,C} (A>=B>=C) of cube blocks, loads W on the (up to) six permuted cubes
+// into LDS (8-element runs along a), forms V = W + T1 x (ia|jb) terms and sums the orbit, so
+// every element of W is read once per triple.
 static constexpr int CT = 8, CT3 = CT * CT * CT;
 
-__global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int j, int k0, TriplesDesc d,
-                                                        const int* __restrict__ orbits, int norb,
+__global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int j, int k0, const double* __restrict__ W,
+                                                        const int* __restrict__ orbits,
                                                         const double* __restrict__ t1p, const double* __restrict__ vvoo,
                                                         const double* __restrict__ eo, const double* __restrict__ ev,
                                                         double* __restrict__ partials) {
@@ -354,46 +354,27 @@ __global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int
       if (pb[q][0] == pb[tid][0] && pb[q][1] == pb[tid][1] && pb[q][2] == pb[tid][2]) { s = q; break; }
     slot_of[tid] = s;
   }
+  __syncthreads();
   double acc = 0.0;
   if (wocc != 0.0) {
-    for (int p = 0; p < 6; ++p)
-      for (int e = tid; e < CT3; e += 256) Wc[p][e] = 0.0;
-    __syncthreads();
     const int64_t vv = (int64_t)v * v;
-    // assemble W on every distinct cube, class by class
-    for (int p = 0; p < 6; ++p) {
-      if (slot_of[p] != p) continue;
-      const int P0 = pb[p][0] * CT, Q0 = pb[p][1] * CT, R0 = pb[p][2] * CT;
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        const int u = q >> 1;  // unit-stride axis of class q: 0 -> a, 1 -> b, 2 -> c
-        const double* G = d.G[q];
-        const int64_t sa = d.sa[q], sb = d.sb[q], sc = d.sc[q], skk = d.sk[q] * kk;
-        for (int e = tid; e < CT3; e += 256) {
-          const int x = e & 7, y = (e >> 3) & 7, z = e >> 6;
-          int la, lb, lc;
-          if (u == 0) { la = x; lb = y; lc = z; }
-          else if (u == 1) { lb = x; la = y; lc = z; }
-          else { lc = x; la = y; lb = z; }
-          const int a = P0 + la, b = Q0 + lb, c = R0 + lc;
-          if (a < v && b < v && c < v) Wc[p][la + CT * lb + CT * CT * lc] += G[a * sa + b * sb + c * sc + skk];
-        }
-        __syncthreads();
-      }
-    }
-    // V = W + T1p[x,i] vvoo[y,z,j,k] + vvoo[x,z,i,k] T1p[y,j] + vvoo[x,y,i,j] T1p[z,k]   (ijk.jl:116)
+    const double* Wk = W + vv * v * kk;
     const double* vjk = vvoo + vv * (j + (int64_t)o * k);
     const double* vik = vvoo + vv * (i + (int64_t)o * k);
     const double* vij = vvoo + vv * (i + (int64_t)o * j);
+    // load W and form V = W + T1p[x,i] vvoo[y,z,j,k] + vvoo[x,z,i,k] T1p[y,j] + vvoo[x,y,i,j] T1p[z,k]  (ijk.jl:116)
     for (int p = 0; p < 6; ++p) {
       if (slot_of[p] != p) continue;
       const int P0 = pb[p][0] * CT, Q0 = pb[p][1] * CT, R0 = pb[p][2] * CT;
       for (int e = tid; e < CT3; e += 256) {
         const int a = P0 + (e & 7), b = Q0 + ((e >> 3) & 7), c = R0 + (e >> 6);
-        double val = 0.0;
-        if (a < v && b < v && c < v)
-          val = Wc[p][e] + t1p[a + (int64_t)v * i] * vjk[b + (int64_t)v * c] + vik[a + (int64_t)v * c] * t1p[b + (int64_t)v * j] +
+        double w = 0.0, val = 0.0;
+        if (a < v && b < v && c < v) {
+          w = Wk[a + (int64_t)v * b + vv * c];
+          val = w + t1p[a + (int64_t)v * i] * vjk[b + (int64_t)v * c] + vik[a + (int64_t)v * c] * t1p[b + (int64_t)v * j] +
                 vij[a + (int64_t)v * b] * t1p[c + (int64_t)v * k];
+        }
+        Wc[p][e] = w;
         Vc[p][e] = val;
       }
     }
@@ -445,13 +426,13 @@ void triples_orbits(int v, std::vector<int>& out) {
       for (int c = 0; c <= b; ++c) { out.push_back(a); out.push_back(b); out.push_back(c); }
 }
 
-void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TriplesDesc& d, const int* orbits,
+void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const double* W, const int* orbits,
                     const double* t1p, const double* vvoo, const double* eo, const double* ev, double* partials,
                     int64_t* nblocks) {
   const int norb = triples_orbit_count(v);
   *nblocks = (int64_t)norb * nk;
-  hipLaunchKernelGGL(k_triples_energy, dim3((unsigned)norb, (unsigned)nk), dim3(256), 0, s, o, v, i, j, k0, d, orbits,
-                     norb, t1p, vvoo, eo, ev, partials);
+  hipLaunchKernelGGL(k_triples_energy, dim3((unsigned)norb, (unsigned)nk), dim3(256), 0, s, o, v, i, j, k0, W, orbits,
+                     t1p, vvoo, eo, ev, partials);
   FH_HIP(hipGetLastError());
 }
 
