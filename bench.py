@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--config", default="water10", help="water1 | water4 | water10 (synthetic DF shapes)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--diis-prec", default="single", choices=["single", "double"])
+    ap.add_argument("--precision", default="mixed", choices=["double", "mixed"],
+                    help="mixed: FP32-MFMA contractions + FP64 polish and FP32 (T) W build; its energies are "
+                         "checked against an FP64 run of the same workload (reported as error_vs_fp64)")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--phase", default="full", choices=["full", "triples", "copy"],
@@ -142,7 +145,8 @@ def main():
     log(f"bench: config {args.config} o={o} v={v} naux={Q} seed={args.seed}")
     eo, ev, Boo, Bov, Bvv = synth.molecular_like(o, v, Q, args.seed)
     ctx = fermihip.Context(0)
-    opts = fermihip.Options(diis_prec=args.diis_prec)
+    opts = fermihip.Options(diis_prec=args.diis_prec, precision=args.precision)
+    tprec = 1 if args.precision == "mixed" else 0
 
     def step():
         t0 = time.perf_counter()
@@ -154,7 +158,7 @@ def main():
             r = None
         else:
             r, _, _ = ctx.solve(opts, copy_out=False)
-        Et = ctx.triples()
+        Et = ctx.triples(prec_mode=tprec)
         t2 = time.perf_counter()
         return t2 - t1, t1 - t0, r, Et
 
@@ -192,33 +196,55 @@ def main():
     kern = {n: rate(n) for n in ph}
     # dominant MFMA kernel of the step: the PP ladder GEMM or the (T) W-build GEMM (distinct
     # kernel symbols via the tag template argument, so rocprofv3 lists them separately)
-    launches_per_call = {"ladder": 1, "triples_gemm": 3}
-    names = {"ladder": "k_gemm_f64<64, 64, 16, false, false, 4, 1> (PP ladder, tag 1)",
-             "triples_gemm": "k_gemm_f64<64, 64, 8, false, false, 4, 2> ((T) W build, tag 2)"}
-    dom = max(("ladder", "triples_gemm"), key=lambda n: (kern[n]["ms"] if kern.get(n) else -1))
-    kd = kern[dom]
-    nl = kd["calls"] * launches_per_call[dom]
-    roof = {"kernel": names[dom], "bound": "mfma", "achieved": kd["tflops"], "peak": FP64_MFMA_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": kd["tflops"] / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
-            "flops_per_launch": kd["flops"] / max(1, nl), "avg_launch_us": kd["ms"] * 1e3 / max(1, nl),
-            "launches": nl}
-    other = "ladder" if dom == "triples_gemm" else "triples_gemm"
-    roof_other = None
-    if kern.get(other):
-        ko = kern[other]
-        no = ko["calls"] * launches_per_call[other]
-        roof_other = {"kernel": names[other], "achieved": ko["tflops"], "frac": ko["tflops"] / FP64_MFMA_PEAK_TFLOPS,
-                      "avg_launch_us": ko["ms"] * 1e3 / max(1, no), "launches": no}
+    FP32_MFMA_PEAK_TFLOPS = 157.3
+    sfx = "_fp32" if args.precision == "mixed" else ""
+    kt = "float" if sfx else "double"
+    launches_per_call = {"ladder": 1, "triples_gemm": 6}
+    names = {"ladder": f"k_gemm_f64<64, 64, 16, false, false, 4, 1, {'true' if sfx else 'false'}> (PP ladder, tag 1, {kt} MFMA)",
+             "triples_gemm": f"k_gemm_f64<64, 64, 8, false, false, 4, 2, {'true' if sfx else 'false'}> ((T) W build, tag 2, {kt} MFMA)"}
+    peak = FP32_MFMA_PEAK_TFLOPS if sfx else FP64_MFMA_PEAK_TFLOPS
+
+    def roof_of(n):
+        kd = kern.get(n + sfx)
+        if not kd:
+            return None
+        nl = kd["calls"] * launches_per_call[n]
+        return {"kernel": names[n], "bound": "mfma", "achieved": kd["tflops"], "peak": peak, "unit": "TFLOP/s",
+                "frac": kd["tflops"] / peak, "traffic": None, "flops_per_launch": kd["flops"] / max(1, nl),
+                "avg_launch_us": kd["ms"] * 1e3 / max(1, nl), "launches": nl}
+
+    cand = [n for n in ("ladder", "triples_gemm") if kern.get(n + sfx)]
+    dom = max(cand, key=lambda n: kern[n + sfx]["ms"])
+    roof = roof_of(dom)
+    roof_other = roof_of("ladder" if dom == "triples_gemm" else "triples_gemm")
+    fp64_ref = None
+    if args.precision == "mixed":
+        # the FP64 GPU path on the same workload: reference energies and its time-to-solution
+        opts64 = fermihip.Options(diis_prec=args.diis_prec, precision="double")
+        ctx.set_df(eo, ev, Boo, Bov, Bvv)
+        t1 = time.perf_counter()
+        ctx.prepare()
+        ctx.guess(copy_out=False)
+        r64, _, _ = ctx.solve(opts64, copy_out=False)
+        Et64 = ctx.triples(prec_mode=0)
+        t64 = time.perf_counter() - t1
+        log(f"bench: FP64 path {t64:.3f} s niter={r64.niter} Ecorr={r64.correlation:.12f} Et={Et64:.12e}")
+        fp64_ref = {"time_to_solution_s": t64, "ccsd_correlation": r64.correlation, "triples": Et64,
+                    "niter": r64.niter, "speedup_mixed_vs_fp64": t64 / tts,
+                    "error_vs_fp64_Eh": {"ccsd": r.correlation - r64.correlation, "triples": Et - Et64,
+                                         "total": (r.correlation + Et) - (r64.correlation + Et64)}}
     out = {
         "metric": "DF-RCCSD(T) time-to-solution (s) + MFMA TFLOP/s at <=1 uEh vs FP64 GPU path",
         "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": tts * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (seeded DF tensors of the cluster's shape)",
-        "config": {"workload": f"DF-RCCSD(T) FP64, {args.config} shape: o={o} v={v} naux={Q} (synthetic "
+        "dtype": "f64" if args.precision == "double" else "f32/f64", "data": "synthetic (seeded DF tensors of the cluster's shape)",
+        "config": {"workload": f"DF-RCCSD(T) {'mixed FP32/FP64' if args.precision == 'mixed' else 'FP64'}, {args.config} shape: o={o} v={v} naux={Q} (synthetic "
                                f"molecular-like DF tensors, seed {args.seed})",
-                   "o": o, "v": v, "naux": Q, "precision": "fp64", "diis_prec": args.diis_prec,
+                   "o": o, "v": v, "naux": Q, "precision": args.precision, "diis_prec": args.diis_prec,
                    "parallelism": "single GPU"},
-        "energies": {"ccsd_correlation": r.correlation, "triples": Et, "niter": r.niter, "converged": bool(r.converged)},
+        "energies": {"ccsd_correlation": r.correlation, "triples": Et, "niter": r.niter, "converged": bool(r.converged),
+                     "niter_fp32": r.niter_mixed},
+        "fp64_path": fp64_ref,
         "pcie_inclusive_s": tts + sum(uploads) / len(uploads),
         "roofline": roof,
         "roofline_second": roof_other,

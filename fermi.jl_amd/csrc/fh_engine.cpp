@@ -99,6 +99,7 @@ struct Ctx {
   std::unique_ptr<Pool> pool;
   int o = 0, v = 0, naux = 0;
   bool have_df = false, prepared = false;
+  bool lowp = false;  // residual contractions as FP32 MFMA (mixed-precision phase)
   Buf eo, ev, Boo, Bov, Bvv;
   // 4-index blocks (Julia index order, column-major)
   Buf OOOO, OOOV, OOVV, OVOV, OVVV, Lovov, Vperm, Pt;
@@ -296,15 +297,15 @@ static void contract(Ctx& c, double alpha, const TView& A, const std::string& ia
   if (wsz) work = dalloc(c, wsz);
   if (cmode == 0) {
     gemm_f64(c.s, transA, transB, Msz, Nsz, Ksz, alpha, pA, lda, 0, pB, ldb, 0, beta, C.p, Msz, 0, C.p, Msz, 0, 1, 0,
-             work.d(), wsz);
+             work.d(), wsz, nullptr, c.lowp);
   } else if (cmode == 1) {
     // C^T (Nsz x Msz) = op(B)^T op(A)^T
     gemm_f64(c.s, !transB, !transA, Nsz, Msz, Ksz, alpha, pB, ldb, 0, pA, lda, 0, beta, C.p, Nsz, 0, C.p, Nsz, 0, 1, 0,
-             work.d(), wsz);
+             work.d(), wsz, nullptr, c.lowp);
   } else {
     tC = dalloc(c, Msz * Nsz);
     gemm_f64(c.s, transA, transB, Msz, Nsz, Ksz, 1.0, pA, lda, 0, pB, ldb, 0, 0.0, nullptr, Msz, 0, tC.d(), Msz, 0, 1,
-             0, work.d(), wsz);
+             0, work.d(), wsz, nullptr, c.lowp);
     TView T;
     T.p = tC.d();
     T.nd = (int)(M.size() + N.size());
@@ -328,6 +329,8 @@ static void require_df(Ctx& c) { FH_CHECK(c.have_df, -6, "no DF tensors: call fh
 static void prepare(Ctx& c) {
   require_df(c);
   if (c.prepared) return;
+  const bool keep = c.lowp;
+  c.lowp = false;  // integrals are assembled in FP64
   const int64_t o = c.o, v = c.v, Q = c.naux;
   Timer t(c, "integrals", 2.0 * Q * (double)(o * o * o * o + o * o * o * v + o * o * v * v + o * v * o * v +
                                               o * v * v * v + v * v * v * v));
@@ -355,6 +358,7 @@ static void prepare(Ctx& c) {
   c.Pt = dalloc(c, v * v * v * o);
   perm(c, tv(c.OVVV.d(), {o, v, v, v}), "kdac", tv(c.Pt.d(), {v, v, v, o}), "cdak", 1.0, 0.0);
   c.prepared = true;
+  c.lowp = keep;
 }
 
 // ------------------------------------------------------------------ CCSD numerators
@@ -440,16 +444,16 @@ static void residual(Ctx& c, const double* t1p, const double* t2p, double* R1, d
     {
       Timer tl(c, "ladder_build", 2.0 * o_ * v_ * v_ * v_ * v_);
       gemm_f64(c.s, false, false, v * v * v, v, o, -1.0, c.Pt.d(), v * v * v, 0, t1p, o, 0, 0.5, c.Vperm.d(),
-               v * v * v, 0, bY.d(), v * v * v, 0);
+               v * v * v, 0, bY.d(), v * v * v, 0, 1, 0, nullptr, 0, nullptr, c.lowp);
     }
-    Timer tl(c, "ladder", 2.0 * o_ * o_ * v_ * v_ * v_ * v_);
+    Timer tl(c, c.lowp ? "ladder_fp32" : "ladder", 2.0 * o_ * o_ * v_ * v_ * v_ * v_);
     gemm_f64(c.s, false, false, o * o, v * v, v * v, 1.0, tau.p, o * o, 0, bY.d(), v * v, 0, 1.0, P.p, o * o, 0, P.p,
-             o * o, 0, 1, /*tag: PP ladder*/ 1);
+             o * o, 0, 1, /*tag: PP ladder*/ 1, nullptr, 0, nullptr, c.lowp);
   }
   contract(c, 1.0, Lvv, "ac", t2, "ijcb", 1.0, P, "ijab");
   contract(c, -1.0, Loo, "ki", t2, "kjab", 1.0, P, "ijab");
   {
-    Timer tr(c, "ring", 6.0 * 2.0 * o_ * o_ * o_ * v_ * v_ * v_);
+    Timer tr(c, c.lowp ? "ring_fp32" : "ring", 6.0 * 2.0 * o_ * o_ * o_ * v_ * v_ * v_);
     Buf bWa = dalloc(c, c.n2()), bWb = dalloc(c, c.n2()), bXX = dalloc(c, c.n2());
     TView Wvoov = tv(bWa.d(), {v, o, o, v}), Wvovo = tv(bWb.d(), {v, o, v, o});
     // Wvoov[a,k,i,c]
@@ -597,6 +601,7 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   FH_CHECK(c.have_amp, -6, "no starting amplitudes");
   FH_CHECK(op.cc_max_iter >= 0 && op.cc_ndiis >= 1 && op.cc_ndiis <= 8 && op.cc_diis_relax >= 1, -1,
            "invalid CC options");
+  FH_CHECK(op.precision_mode == 0 || op.precision_mode == 1, -1, "precision_mode must be 0 (FP64) or 1 (FP32 mixed)");
   const int64_t n1 = c.nov(), n2 = c.n2();
   const double epsT = 2.220446049250313e-16;
   double e_conv = op.cc_e_conv, max_rms = op.cc_max_rms;
@@ -604,8 +609,13 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   const bool do_diis = op.cc_diis != 0;
   const bool f32 = op.diis_prec != 0;
   const double dp = op.cc_damp_ratio;
-  DIIS dm1(c, n1, op.cc_ndiis, f32), dm2(c, n2, op.cc_ndiis, f32);
+  std::unique_ptr<DIIS> dm1(new DIIS(c, n1, op.cc_ndiis, f32)), dm2(new DIIS(c, n2, op.cc_ndiis, f32));
   c.iters.clear();
+  // mixed precision: phase 1 runs the residual's contractions in FP32 until the switch
+  // thresholds are met, then the FP64 phase polishes (fresh DIIS subspace)
+  bool mixed = op.precision_mode == 1;
+  int polish_done = 0, niter_mixed = 0;
+  const double sw_e = std::max(op.mixed_e_switch, e_conv), sw_rms = std::max(op.mixed_rms_switch, max_rms);
 
   Buf nT1 = std::move(c.T1), nT2 = std::move(c.T2);     // newT (guess)
   Buf oT1 = dalloc(c, n1), oT2 = dalloc(c, n2);         // T
@@ -614,8 +624,20 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   double dE = 1.0, rms = 1.0;
   int ite = 1;
   if (op.verbose) std::printf("%10s    %15s    %12s    %12s    %10s    %5s\n", "Iteration", "CC Energy", "dE", "Max RMS", "time (s)", "diis");
-  while (std::fabs(dE) > e_conv || rms > max_rms) {                                             // :178
+  while (true) {
+    if (mixed && std::fabs(dE) < sw_e && rms < sw_rms) {
+      mixed = false;                                       // switch to FP64 polish
+      dm1.reset(new DIIS(c, n1, op.cc_ndiis, f32));
+      dm2.reset(new DIIS(c, n2, op.cc_ndiis, f32));
+      dE = 1.0;
+      rms = 1.0;
+    }
+    if (!mixed) {
+      if (!(std::fabs(dE) > e_conv || rms > max_rms)) break;                                   // :178
+      if (op.precision_mode == 1 && op.polish_iters >= 0 && polish_done >= op.polish_iters) break;
+    }
     if (ite > op.cc_max_iter) break;
+    c.lowp = mixed;
     auto t0 = std::chrono::steady_clock::now();
     std::swap(oT1, nT1);                                                                        // :185-186
     std::swap(oT2, nT2);
@@ -629,13 +651,15 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
     }
     double r1 = 0.0, r2 = 0.0;
     update_amp(c, oT1.d(), oT2.d(), nT1.d(), nT2.d(), e1.p, e2.p, f32, &r1, &r2);               // :190-194
+    c.lowp = false;
     bool diis = false;
     if (do_diis) {
-      dm1.push(nT1.d(), std::move(e1));                                                         // :199-202
-      dm2.push(nT2.d(), std::move(e2));
-      if (ite >= op.diis_start && ite % op.cc_diis_relax == 0) {                                // :204
-        dm2.extrapolate(nT2.d(), true);                                                         // :206
-        dm1.extrapolate(nT1.d(), true);                                                         // :207
+      dm1->push(nT1.d(), std::move(e1));                                                        // :199-202
+      dm2->push(nT2.d(), std::move(e2));
+      if (ite >= op.diis_start && ite % op.cc_diis_relax == 0 &&                             // :204
+          (op.precision_mode == 0 || dm2->vecs.size() >= 2)) {
+        dm2->extrapolate(nT2.d(), true);                                                        // :206
+        dm1->extrapolate(nT1.d(), true);                                                        // :207
         diis = true;
       }
     }
@@ -646,17 +670,21 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
     Ecc = energy(c, nT1.d(), nT2.d());                                                          // :218
     rms = std::max(r1, r2);
     dE = Ecc - oldE;
+    if (mixed) ++niter_mixed;
+    else if (op.precision_mode == 1) ++polish_done;
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c.iters.push_back({ite, Ecc, dE, rms, ms, diis});
-    if (op.verbose) std::printf("    %-5d    %-15.10f    %+12.10f    %-12.10f    %-10.5f    %s\n", ite, Ecc, dE, rms, ms / 1e3, diis ? "true" : "false");
+    if (op.verbose) std::printf("    %-5d    %-15.10f    %+12.10f    %-12.10f    %-10.5f    %s%s\n", ite, Ecc, dE, rms, ms / 1e3, diis ? "true" : "false", mixed ? " fp32" : "");
     FH_CHECK(std::isfinite(Ecc) && std::isfinite(rms), -5, "non-finite CC energy or amplitudes at iteration " + std::to_string(ite));
     ++ite;
   }
+  c.lowp = false;
   res.correlation = Ecc;
   res.e_conv = dE;
   res.t_conv = rms;
   res.niter = ite - 1;
-  res.converged = (std::fabs(dE) < e_conv && rms < max_rms) ? 1 : 0;                          // :231
+  res.niter_mixed = niter_mixed;
+  res.converged = (!mixed && std::fabs(dE) < e_conv && rms < max_rms) ? 1 : 0;                 // :231
   c.T1 = std::move(nT1);
   c.T2 = std::move(nT2);
   c.have_amp = true;
@@ -713,13 +741,13 @@ __global__ void k_accumulate(int64_t n, const double* __restrict__ partials, dou
   if (threadIdx.x == 0) *acc += red[0];
 }
 
-static double triples(Ctx& c) {
+static double triples(Ctx& c, bool lowp) {
   prepare(c);
   FH_CHECK(c.have_amp, -6, "no amplitudes for (T)");
   const int64_t o = c.o, v = c.v, vv = v * v, nd = v + o;
   const double o_ = (double)o, v_ = (double)v;
   const double ntrip = o_ * (o_ + 1) * (o_ + 2) / 6.0;
-  Timer tall(c, "triples", ntrip * 12.0 * (v_ * v_ * v_ * v_ + o_ * v_ * v_ * v_));
+  Timer tall(c, lowp ? "triples_fp32" : "triples", ntrip * 12.0 * (v_ * v_ * v_ * v_ + o_ * v_ * v_ * v_));
   // AV[(x,y),n,d'] and the row-swapped copy AVs[(y,x),n,d'] share one allocation;
   // BVq (Bcat_{p,q} stored [d',z,q,p]) and BVp (stored [d',z,p,q]) likewise
   const int64_t avsz = vv * o * nd, bvsz = nd * v * o * o;
@@ -765,27 +793,28 @@ static double triples(Ctx& c) {
       const int64_t kend = (i == j) ? j : j + 1;  // i=j=k has zero weight (ijk.jl:133)
       for (int64_t k0 = 0; k0 < kend; k0 += KC) {
         const int64_t nk = std::min<int64_t>(KC, kend - k0);
-        std::unique_ptr<Timer> tg(new Timer(c, "triples_gemm", 12.0 * (double)vv * (double)v * (double)nd * (double)nk));
+        std::unique_ptr<Timer> tg(new Timer(c, lowp ? "triples_gemm_fp32" : "triples_gemm",
+                                            12.0 * (double)vv * (double)v * (double)nd * (double)nk));
         // The six permutation classes of ijk2.jl:109-153, G = AV_n (rows (x,y)) * Bcat_{p,q}
         // (cols z), accumulated into W[a,b,c,kk] (W[a,b,c] += G[class-permuted (a,b,c)]).
         // abc: n=j, B_{i,k}, x=a y=b z=c
         gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AV + vv * j, vv * o, 0, BVq + bstride * (k0 + o * i), nd, 0,
-                 0.0, nullptr, vv, 0, W, vv, 0, 1, 2);
+                 0.0, nullptr, vv, 0, W, vv, 0, 1, 2, nullptr, 0, nullptr, lowp);
         // bac: n=i, B_{j,k}, x=b y=a z=c  (row-swapped AV: rows (a,b))
         gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVs + vv * i, vv * o, 0, BVq + bstride * (k0 + o * j), nd, 0,
-                 1.0, W, vv, 0, W, vv, 0, 1, 2);
+                 1.0, W, vv, 0, W, vv, 0, 1, 2, nullptr, 0, nullptr, lowp);
         // cab: n=i, B_{k,j}, x=c y=a z=b  (rows (a,c), cols (b,kk))
         gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVs + vv * i, vv * o, 0, BVp + bstride * (k0 + o * j), nd, 0,
-                 1.0, W, vv, 0, W, vv, 0, 1, 2, nullptr, 0, &m_cab);
+                 1.0, W, vv, 0, W, vv, 0, 1, 2, nullptr, 0, &m_cab, lowp);
         // cba: n=j, B_{k,i}, x=c y=b z=a  (rows (b,c), cols (a,kk))
         gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVs + vv * j, vv * o, 0, BVp + bstride * (k0 + o * i), nd, 0,
-                 1.0, W, vv, 0, W, vv, 0, 1, 2, nullptr, 0, &m_cba);
+                 1.0, W, vv, 0, W, vv, 0, 1, 2, nullptr, 0, &m_cba, lowp);
         // acb: n=k, B_{i,j}, x=a y=c z=b  (batched over k: rows (a,c), col b)
         gemm_f64(c.s, false, false, vv, v, nd, 1.0, AV + vv * k0, vv * o, vv, BVp + bstride * (i + o * j), nd, 0, 1.0, W,
-                 vv, v3, W, vv, v3, nk, 2, nullptr, 0, &m_acb);
+                 vv, v3, W, vv, v3, nk, 2, nullptr, 0, &m_acb, lowp);
         // bca: n=k, B_{j,i}, x=b y=c z=a  (batched over k: rows (b,c), col a)
         gemm_f64(c.s, false, false, vv, v, nd, 1.0, AV + vv * k0, vv * o, vv, BVp + bstride * (j + o * i), nd, 0, 1.0, W,
-                 vv, v3, W, vv, v3, nk, 2, nullptr, 0, &m_bca);
+                 vv, v3, W, vv, v3, nk, 2, nullptr, 0, &m_bca, lowp);
         tg.reset();
         Timer te(c, "triples_energy", 0.0);
         int64_t nb = 0;
@@ -844,6 +873,10 @@ void fh_default_opts(fh_cc_opts* p) {
   p->cc_damp_ratio = 0.0;
   p->precision_override = 0;
   p->verbose = 0;
+  p->precision_mode = 0;
+  p->polish_iters = -1;
+  p->mixed_e_switch = 1e-8;
+  p->mixed_rms_switch = 1e-7;
 }
 
 const char* fh_last_error(void) { return g_last_error.c_str(); }
@@ -1007,13 +1040,13 @@ int fh_ccsd_solve(fh_ctx* x, const fh_cc_opts* opts, double* T1, double* T2, fh_
 int fh_triples(fh_ctx* x, int prec_mode, const double* T1, const double* T2, double* Et) {
   return guarded([&] {
     FH_CHECK(x && Et, -1, "null argument");
-    FH_CHECK(prec_mode == 0, -1, "fh_triples: only prec_mode 0 (FP64) is available");
+    FH_CHECK(prec_mode == 0 || prec_mode == 1, -1, "fh_triples: prec_mode must be 0 (FP64) or 1 (FP32 W build)");
     FH_CHECK((T1 == nullptr) == (T2 == nullptr), -1, "T1 and T2 must both be given or both NULL");
     DeviceGuard g(x->c.device);
     Ctx& c = x->c;
     require_df(c);
     if (T1) upload_amp(c, T1, T2);
-    *Et = triples(c);
+    *Et = triples(c, prec_mode == 1);
     flush_timers(c);
     FH_CHECK(std::isfinite(*Et), -5, "non-finite (T) energy");
   });

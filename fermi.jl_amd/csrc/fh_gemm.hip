@@ -13,21 +13,30 @@
 #include "fh_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fh {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-template <int BM, int BN, int BK, bool TA, bool TB, int OCC, int TAG>
+// F32 = false: FP64 operands in LDS, v_mfma_f64_16x16x4f64, FP64 accumulation.
+// F32 = true : the FP64 operands are rounded to FP32 while staged into LDS,
+//              v_mfma_f32_16x16x4f32 with FP32 accumulation, alpha/beta epilogue in FP64
+//              (the mixed-precision path's bulk contractions).
+template <int BM, int BN, int BK, bool TA, bool TB, int OCC, int TAG, bool F32>
 __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
     int64_t M, int64_t N, int64_t K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
     const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
     int64_t sC, double* D, int64_t ldd, int64_t sD, int tilesM, int tilesN, int64_t kchunk, int nfast, OutMap om) {
-  constexpr int PADA = 16, PADB = 2;
+  using T = typename std::conditional<F32, float, double>::type;
+  using Acc = typename std::conditional<F32, f32x4, f64x4>::type;
+  // paddings: a half (FP64) or full (FP32) wavefront's fragment reads hit distinct banks
+  constexpr int PADA = 16, PADB = F32 ? 4 : 2;
   constexpr int WM = BM / 2, WN = BN / 2, MB = WM / 16, NB = WN / 16;
   constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
-  __shared__ double As[2][BK][BM + PADA];
-  __shared__ double Bs[2][BN][BK + PADB];
+  __shared__ T As[2][BK][BM + PADA];
+  __shared__ T Bs[2][BN][BK + PADB];
 
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wm = wid & 1, wn = wid >> 1;
@@ -85,22 +94,22 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
       const int e = tid + 256 * t;
       int mm, kk;
       if (!TA) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
-      As[buf][kk][mm] = ra[t];
+      As[buf][kk][mm] = (T)ra[t];
     }
 #pragma unroll
     for (int t = 0; t < LB; ++t) {
       const int e = tid + 256 * t;
       int nn, kk;
       if (!TB) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
-      Bs[buf][nn][kk] = rb[t];
+      Bs[buf][nn][kk] = (T)rb[t];
     }
   };
 
-  f64x4 acc[MB][NB];
+  Acc acc[MB][NB];
 #pragma unroll
   for (int i = 0; i < MB; ++i)
 #pragma unroll
-    for (int j = 0; j < NB; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < NB; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
   const int64_t nst = (K + BK - 1) / BK;
   gload(0);
@@ -111,7 +120,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
     if (s + 1 < nst) gload((s + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
-      double af[MB], bf[NB];
+      T af[MB], bf[NB];
 #pragma unroll
       for (int i = 0; i < MB; ++i) af[i] = As[buf][kk * 4 + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
 #pragma unroll
@@ -119,13 +128,17 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
 #pragma unroll
       for (int i = 0; i < MB; ++i)
 #pragma unroll
-        for (int j = 0; j < NB; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NB; ++j) {
+          if constexpr (F32) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+          else acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
     }
     if (s + 1 < nst) sstore(buf ^ 1);
     __syncthreads();
   }
 
-  // epilogue: lane holds rows (lane>>4)+4r, column lane&15 of each 16x16 block.  Output
+  // epilogue: lane holds column lane&15 and rows (lane>>4)+4r (FP64 MFMA) or 4(lane>>4)+r
+  // (FP32 MFMA) of each 16x16 block.  Output
   // address: column-major with ldd, or (om.rmod > 0) the two-mode map
   // row r -> (r % rmod)*rs0 + (r / rmod)*rs1, column c -> (c % cmod)*cs0 + (c / cmod)*cs1
   // (cmod = 0: c*cs0), which lets a GEMM accumulate into a permuted 3-/4-index tensor.
@@ -140,10 +153,10 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
     for (int i = 0; i < MB; ++i)
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int64_t row = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * rr;
+        const int64_t row = m0 + wm * WM + i * 16 + (F32 ? 4 * (lane >> 4) + rr : (lane >> 4) + 4 * rr);
         if (row < M) {
           const int64_t roff = om.rmod > 0 ? (row % om.rmod) * om.rs0 + (row / om.rmod) * om.rs1 : row;
-          double val = alpha * acc[i][j][rr];
+          double val = alpha * (double)acc[i][j][rr];
           if (beta != 0.0) val += beta * (om.rmod > 0 ? C[roff + coff] : C[row + ldc * col]);
           D[roff + coff] = val;
         }
@@ -151,7 +164,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
   }
 }
 
-template <int BM, int BN, int BK, bool TA, bool TB, int TAG>
+template <int BM, int BN, int BK, bool TA, bool TB, int TAG, bool F32>
 static void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                    int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
                    int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om) {
@@ -159,12 +172,12 @@ static void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha,
   FH_CHECK(tm * tn < (int64_t)1 << 31 && batch < 65536, -3, "gemm grid too large");
   dim3 grid((unsigned)(tm * tn), (unsigned)batch);
   const int nfast = (M > N) ? 1 : 0;   // stream the larger of A (M x K) and B (K x N) once
-  hipLaunchKernelGGL((k_gemm_f64<BM, BN, BK, TA, TB, (BM * BN > 8192 ? 1 : 4), TAG>), grid, dim3(256), 0, s, M, N, K, alpha, A,
+  hipLaunchKernelGGL((k_gemm_f64<BM, BN, BK, TA, TB, (BM * BN > 8192 ? 1 : 4), TAG, F32>), grid, dim3(256), 0, s, M, N, K, alpha, A,
                      lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, nfast, om);
   FH_HIP(hipGetLastError());
 }
 
-template <bool TA, bool TB, int TAG>
+template <bool TA, bool TB, int TAG, bool F32>
 static void pick(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                  int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
                  int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om) {
@@ -173,9 +186,9 @@ static void pick(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, c
   // BK = 8 when the per-tile K loop is short.
   const int64_t kl = kchunk > 0 ? kchunk : K;
   if (kl <= 512)
-    launch<64, 64, 8, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk, om);
+    launch<64, 64, 8, TA, TB, TAG, F32>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk, om);
   else
-    launch<64, 64, 16, TA, TB, TAG>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk, om);
+    launch<64, 64, 16, TA, TB, TAG, F32>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk, om);
 }
 
 // split-K sizing: few output tiles and a long K leave most CUs idle; cut K into slices
@@ -209,7 +222,7 @@ __global__ void k_splitk_reduce(int64_t M, int64_t N, int64_t S, const double* _
 void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
               int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C,
               int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int tag, double* work,
-              int64_t work_elems, const OutMap* map) {
+              int64_t work_elems, const OutMap* map, bool f32) {
   const OutMap om = map ? *map : OutMap{};
   FH_CHECK(!map || (map->rmod > 0 && (beta == 0.0 || C == D)), -3, "gemm: mapped output needs rmod > 0 and C == D");
   if (M <= 0 || N <= 0 || batch <= 0) return;
@@ -225,11 +238,16 @@ void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, 
   const double* Cc = split ? nullptr : C;
   double* Dd = split ? work : D;
   const int64_t lddd = split ? M : ldd, sDD = split ? M * N : sD;
-#define FH_PICK(TA_, TB_)                                                                                            \
+#define FH_PICK2(TA_, TB_, F_)                                                                                  \
   do {                                                                                                              \
-    if (tag == 1) pick<TA_, TB_, 1>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om); \
-    else if (tag == 2) pick<TA_, TB_, 2>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om); \
-    else pick<TA_, TB_, 0>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om);         \
+    if (tag == 1) pick<TA_, TB_, 1, F_>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om); \
+    else if (tag == 2) pick<TA_, TB_, 2, F_>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om); \
+    else pick<TA_, TB_, 0, F_>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om);         \
+  } while (0)
+#define FH_PICK(TA_, TB_)                   \
+  do {                                      \
+    if (f32) FH_PICK2(TA_, TB_, true);      \
+    else FH_PICK2(TA_, TB_, false);         \
   } while (0)
   if (ta) {
     if (tb) FH_PICK(true, true); else FH_PICK(true, false);
@@ -237,6 +255,7 @@ void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, 
     if (tb) FH_PICK(false, true); else FH_PICK(false, false);
   }
 #undef FH_PICK
+#undef FH_PICK2
   if (split) {
     int64_t g = (M * N + 255) / 256;
     if (g > 65536) g = 65536;

@@ -93,7 +93,8 @@ void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, 
               const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB,
               double beta, const double* C, int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD,
               int64_t batch = 1, int tag = 0, double* work = nullptr, int64_t work_elems = 0,
-              const OutMap* map = nullptr);
+              const OutMap* map = nullptr, bool f32 = false);
+// f32 = true: operands rounded to FP32 in LDS, FP32 MFMA with FP32 accumulation (mixed precision)
 // workspace (doubles) that gemm_f64 would use to split K for this shape (0: no split)
 int64_t gemm_splitk_elems(int64_t M, int64_t N, int64_t K, int64_t batch);
 // tag selects a distinct kernel symbol for profiling: 0 generic, 1 PP ladder, 2 (T) W build

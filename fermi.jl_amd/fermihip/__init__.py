@@ -44,6 +44,8 @@ class _Opts(ctypes.Structure):
         ("cc_diis", ctypes.c_int), ("cc_ndiis", ctypes.c_int), ("diis_start", ctypes.c_int),
         ("cc_diis_relax", ctypes.c_int), ("diis_prec", ctypes.c_int), ("cc_damp_ratio", ctypes.c_double),
         ("precision_override", ctypes.c_int), ("verbose", ctypes.c_int),
+        ("precision_mode", ctypes.c_int), ("polish_iters", ctypes.c_int), ("mixed_e_switch", ctypes.c_double),
+        ("mixed_rms_switch", ctypes.c_double),
     ]
 
 
@@ -51,6 +53,7 @@ class _Result(ctypes.Structure):
     _fields_ = [
         ("guess_energy", ctypes.c_double), ("correlation", ctypes.c_double), ("e_conv", ctypes.c_double),
         ("t_conv", ctypes.c_double), ("niter", ctypes.c_int), ("converged", ctypes.c_int),
+        ("niter_mixed", ctypes.c_int),
     ]
 
 
@@ -105,15 +108,20 @@ def _colmajor(a):
 
 
 def Options(**kw):
-    """Fermi option keys read on this path, with the reference defaults (Options.jl:62-102)."""
+    """Fermi option keys read on this path, with the reference defaults (Options.jl:62-102), plus
+    the mixed-precision keys of this backend: precision ("double" | "mixed": FP32 contractions
+    then FP64 polish), polish_iters (-1: polish to convergence), mixed_e_switch, mixed_rms_switch."""
     o = dict(cc_max_iter=50, cc_e_conv=1e-10, cc_max_rms=1e-10, cc_diis=True, cc_ndiis=3, diis_start=3,
-             cc_diis_relax=3, diis_prec="single", cc_damp_ratio=0.0, precision_override=False, verbose=False)
+             cc_diis_relax=3, diis_prec="single", cc_damp_ratio=0.0, precision_override=False, verbose=False,
+             precision="double", polish_iters=-1, mixed_e_switch=1e-8, mixed_rms_switch=1e-7)
     for k, v in kw.items():
         if k not in o:
             raise FermiException(f"unknown option {k}")
         o[k] = v
     if o["diis_prec"] not in ("single", "double"):
         raise FermiException(f"invalid diis_prec {o['diis_prec']!r} (single|double)")
+    if o["precision"] not in ("double", "mixed"):
+        raise FermiException(f"invalid precision {o['precision']!r} (double|mixed)")
     return o
 
 
@@ -213,6 +221,10 @@ class Context:
         c.cc_diis_relax = int(opts["cc_diis_relax"]); c.diis_prec = 1 if opts["diis_prec"] == "single" else 0
         c.cc_damp_ratio = float(opts["cc_damp_ratio"]); c.precision_override = int(bool(opts["precision_override"]))
         c.verbose = int(bool(opts["verbose"]))
+        c.precision_mode = 1 if opts["precision"] == "mixed" else 0
+        c.polish_iters = int(opts["polish_iters"])
+        c.mixed_e_switch = float(opts["mixed_e_switch"])
+        c.mixed_rms_switch = float(opts["mixed_rms_switch"])
         r = _Result()
         if T1 is None:
             self._check(self.L.fh_ccsd_solve(self.h, ctypes.byref(c), None, None, ctypes.byref(r)))

@@ -44,6 +44,10 @@ struct CCOpts
     cc_damp_ratio::Cdouble
     precision_override::Cint
     verbose::Cint
+    precision_mode::Cint
+    polish_iters::Cint
+    mixed_e_switch::Cdouble
+    mixed_rms_switch::Cdouble
 end
 
 struct CCResult
@@ -53,6 +57,7 @@ struct CCResult
     t_conv::Cdouble
     niter::Cint
     converged::Cint
+    niter_mixed::Cint
 end
 
 mutable struct Context
@@ -96,7 +101,8 @@ function cc_opts()
     prec in ("single", "double") || throw(Fermi.Options.FermiException("diis_prec $prec is not supported by RCCSDhip"))
     CCOpts(Options.get("cc_max_iter"), Options.get("cc_e_conv"), Options.get("cc_max_rms"), Options.get("cc_diis"),
            Options.get("cc_ndiis"), Options.get("diis_start"), Options.get("cc_diis_relax"), prec == "single" ? 1 : 0,
-           Options.get("cc_damp_ratio"), Options.get("precision_override"), 0)
+           Options.get("cc_damp_ratio"), Options.get("precision_override"), 0,
+           0, -1, 1e-8, 1e-7)   # FP64 contractions; the mixed mode is selected by the Python/C callers
 end
 
 # RCCSD(moints, aoints, ::RCCSDa) for DF ERIs (RCCSDa.jl:37-44): only the three B blocks and F are needed

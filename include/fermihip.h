@@ -40,6 +40,16 @@ typedef struct fh_cc_opts {
   double cc_damp_ratio;
   int precision_override;
   int verbose;            /* 1: print the iteration table (RCCSDa.jl:177,223) to stdout */
+  /* Mixed precision (no counterpart in the reference, whose only knob is all-FP32 CC):
+   * precision_mode 0 = FP64 contractions throughout; 1 = the residual's contractions run as
+   * FP32 MFMA with FP32 accumulation (amplitudes, denominators, DIIS and energies stay FP64)
+   * until |dE| < mixed_e_switch and rms < mixed_rms_switch, then FP64 polish iterations follow:
+   * polish_iters < 0 iterates in FP64 to the cc_e_conv / cc_max_rms criteria, polish_iters >= 0
+   * stops after that many FP64 iterations. */
+  int precision_mode;
+  int polish_iters;
+  double mixed_e_switch;
+  double mixed_rms_switch;
 } fh_cc_opts;
 
 /* Result of fh_ccsd_solve: the scalar fields of Fermi's RCCSD{T} (RCCSD.jl:61-69),
@@ -51,6 +61,7 @@ typedef struct fh_cc_result {
   double t_conv;          /* last max(rms(dT1), rms(dT2)) */
   int niter;
   int converged;
+  int niter_mixed;        /* iterations run with FP32 contractions (precision_mode 1) */
 } fh_cc_result;
 
 /* Fill opts with Fermi's defaults. */
@@ -88,7 +99,8 @@ int fh_ccsd_residual(fh_ctx* ctx, const double* T1, const double* T2, double* ne
  * fh_triples. */
 int fh_ccsd_solve(fh_ctx* ctx, const fh_cc_opts* opts, double* T1, double* T2, fh_cc_result* res);
 
-/* (T) correction RCCSDpT(ccsd, moints, ::ijk), ijk.jl:20-150.  prec_mode 0 = FP64.
+/* (T) correction RCCSDpT(ccsd, moints, ::ijk), ijk.jl:20-150.  prec_mode 0 = FP64; 1 = the W
+ * build runs as FP32 MFMA with FP32 accumulation, V, denominators and the energy sum stay FP64.
  * T1/T2 NULL: use the device-resident amplitudes of the last solve. */
 int fh_triples(fh_ctx* ctx, int prec_mode, const double* T1, const double* T2, double* Et);
 
