@@ -200,8 +200,11 @@ def main():
     sfx = "_fp32" if args.precision == "mixed" else ""
     kt = "float" if sfx else "double"
     launches_per_call = {"ladder": 1, "triples_gemm": 6}
-    names = {"ladder": f"k_gemm_f64<64, 64, 16, false, false, 4, 1, {'true' if sfx else 'false'}> (PP ladder, tag 1, {kt} MFMA)",
-             "triples_gemm": f"k_gemm_f64<64, 64, 8, false, false, 4, 2, {'true' if sfx else 'false'}> ((T) W build, tag 2, {kt} MFMA)"}
+    f32 = "true" if sfx else "false"
+    ti = "float" if sfx else "double"
+    # template arguments: BM, BN, BK, TA, TB, waves-per-CU hint, tag, FP32 MFMA, operand type, output type
+    names = {"ladder": f"k_gemm<64, 64, 16, false, false, 4, 1, {f32}, {ti}, double> (PP ladder, tag 1, {kt} MFMA)",
+             "triples_gemm": f"k_gemm<64, 64, 8, false, false, 4, 2, {f32}, {ti}, {ti}> ((T) W build, tag 2, {kt} MFMA)"}
     peak = FP32_MFMA_PEAK_TFLOPS if sfx else FP64_MFMA_PEAK_TFLOPS
 
     def roof_of(n):

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <memory>
 #include <sstream>
+#include <type_traits>
 
 namespace fh {
 
@@ -96,6 +97,7 @@ struct Pending {
 struct Ctx {
   int device = 0;
   hipStream_t s = nullptr;
+  // declared before every Buf member, so that it is destroyed after all of them
   std::unique_ptr<Pool> pool;
   int o = 0, v = 0, naux = 0;
   bool have_df = false, prepared = false;
@@ -103,6 +105,7 @@ struct Ctx {
   Buf eo, ev, Boo, Bov, Bvv;
   // 4-index blocks (Julia index order, column-major)
   Buf OOOO, OOOV, OOVV, OVOV, OVVV, Lovov, Vperm, Pt;
+  Buf Vperm32;  // FP32 copy of Vperm for the mixed-precision ladder (made on first use)
   // device-resident amplitudes (last guess / solve)
   Buf T1, T2;
   bool have_amp = false;
@@ -352,6 +355,7 @@ static void prepare(Ctx& c) {
   perm(c, ovov, "kcld", tv(c.Lovov.d(), {o, v, o, v}), "kcld", 2.0, 0.0);
   perm(c, ovov, "kdlc", tv(c.Lovov.d(), {o, v, o, v}), "kcld", -1.0, 1.0);
   // ladder operand Vperm[c,d,a,b] = (ca|db) = Bvv[Q,c,a] Bvv[Q,d,b]  (DFERI.jl:194 permuted)
+  c.Vperm32.reset();  // its FP32 copy is remade from the new Vperm on first use
   c.Vperm = dalloc(c, v * v * v * v);
   contract(c, 1.0, Bvv, "Qca", Bvv, "Qdb", 0.0, tv(c.Vperm.d(), {v, v, v, v}), "cdab");
   // Pt[c,d,a,k] = (kd|ac) = OVVV[k,d,a,c]
@@ -440,15 +444,31 @@ static void residual(Ctx& c, const double* t1p, const double* t2p, double* R1, d
   {
     // particle-particle ladder: P += tau[i,j,c,d] * Y[c,d,a,b],
     // Y = 0.5*(ca|db) - sum_k (kd|ac) t1[k,b]   (Wvvvv with the symmetric half folded into P)
-    Buf bY = dalloc(c, v * v * v * v);
-    {
-      Timer tl(c, "ladder_build", 2.0 * o_ * v_ * v_ * v_ * v_);
-      gemm_f64(c.s, false, false, v * v * v, v, o, -1.0, c.Pt.d(), v * v * v, 0, t1p, o, 0, 0.5, c.Vperm.d(),
-               v * v * v, 0, bY.d(), v * v * v, 0, 1, 0, nullptr, 0, nullptr, c.lowp);
+    const int64_t v3 = v * v * v, v4 = v3 * v;
+    if (!c.lowp) {
+      Buf bY = dalloc(c, v4);
+      {
+        Timer tl(c, "ladder_build", 2.0 * o_ * v_ * v_ * v_ * v_);
+        gemm_f64(c.s, false, false, v3, v, o, -1.0, c.Pt.d(), v3, 0, t1p, o, 0, 0.5, c.Vperm.d(), v3, 0, bY.d(), v3, 0);
+      }
+      Timer tl(c, "ladder", 2.0 * o_ * o_ * v_ * v_ * v_ * v_);
+      gemm_f64(c.s, false, false, o * o, v * v, v * v, 1.0, tau.p, o * o, 0, bY.d(), v * v, 0, 1.0, P.p, o * o, 0, P.p,
+               o * o, 0, 1, /*tag: PP ladder*/ 1);
+    } else {
+      // mixed precision: Y and tau in FP32 in HBM, FP32 MFMA, FP64 accumulation into P
+      if (!c.Vperm32.p) {
+        c.Vperm32 = Buf(c.pool.get(), (size_t)v4 * sizeof(float));
+        cast_f64_to_f32(c.s, v4, c.Vperm.d(), c.Vperm32.f());
+      }
+      Buf bY(c.pool.get(), (size_t)v4 * sizeof(float)), tau32(c.pool.get(), (size_t)c.n2() * sizeof(float));
+      cast_f64_to_f32(c.s, c.n2(), tau.p, tau32.f());
+      {
+        Timer tl(c, "ladder_build_fp32", 2.0 * o_ * v_ * v_ * v_ * v_);
+        gemm_f32_df(c.s, v3, v, o, -1.0, c.Pt.d(), v3, t1p, o, 0.5, c.Vperm32.f(), v3, bY.f(), v3);
+      }
+      Timer tl(c, "ladder_fp32", 2.0 * o_ * o_ * v_ * v_ * v_ * v_);
+      gemm_f32_fd(c.s, o * o, v * v, v * v, 1.0, tau32.f(), o * o, bY.f(), v * v, 1.0, P.p, o * o, P.p, o * o, 1);
     }
-    Timer tl(c, c.lowp ? "ladder_fp32" : "ladder", 2.0 * o_ * o_ * v_ * v_ * v_ * v_);
-    gemm_f64(c.s, false, false, o * o, v * v, v * v, 1.0, tau.p, o * o, 0, bY.d(), v * v, 0, 1.0, P.p, o * o, 0, P.p,
-             o * o, 0, 1, /*tag: PP ladder*/ 1, nullptr, 0, nullptr, c.lowp);
   }
   contract(c, 1.0, Lvv, "ac", t2, "ijcb", 1.0, P, "ijab");
   contract(c, -1.0, Loo, "ki", t2, "kjab", 1.0, P, "ijab");
@@ -692,25 +712,29 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
 }
 
 // ------------------------------------------------------------------ (T), ijk.jl:20-150
-__global__ void k_build_av(int o, int v, const double* __restrict__ ovvv, const double* __restrict__ t2,
-                           double* __restrict__ av) {
+template <typename TO>
+__global__ void k_build_av(int o, int v, bool swap_xy, const double* __restrict__ ovvv, const double* __restrict__ t2,
+                           TO* __restrict__ av) {
   // AV[(x,y), n, d'] : d' < v  -> (n y | x d') = OVVV[n, y, x, d']   (ijk2.jl:33 vvvo layout)
   //                    d' >= v -> -T2[n, l, y, x], l = d' - v       (T2_n with the GEMM's minus sign)
+  // swap_xy stores the rows as (y,x) instead
   const int64_t vv = (int64_t)v * v, nd = (int64_t)v + o;
   const int64_t total = vv * o * nd;
   for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
-    const int64_t x = idx % v, y = (idx / v) % v, n = (idx / vv) % o, dp = idx / (vv * o);
+    const int64_t r0 = idx % v, r1 = (idx / v) % v, n = (idx / vv) % o, dp = idx / (vv * o);
+    const int64_t x = swap_xy ? r1 : r0, y = swap_xy ? r0 : r1;
     double val;
     if (dp < v) val = ovvv[n + (int64_t)o * (y + (int64_t)v * (x + (int64_t)v * dp))];
     else {
       const int64_t l = dp - v;
       val = -t2[n + (int64_t)o * (l + (int64_t)o * (y + (int64_t)v * x))];
     }
-    av[idx] = val;
+    av[idx] = (TO)val;
   }
 }
+template <typename TO>
 __global__ void k_build_bv(int o, int v, bool swap_pq, const double* __restrict__ t2, const double* __restrict__ ooov,
-                           double* __restrict__ bv) {
+                           TO* __restrict__ bv) {
   // Bcat_{p,q}[d', z]: d' < v -> T2p[d', z, p, q] = T2[q, p, z, d'];  d' >= v -> ovoo[l, z, p, q] = OOOV[l, p, q, z]
   // stored as bv[d', z, p, q] (swap_pq = false) or bv[d', z, q, p] (swap_pq = true)
   const int64_t nd = (int64_t)v + o;
@@ -724,7 +748,7 @@ __global__ void k_build_bv(int o, int v, bool swap_pq, const double* __restrict_
       const int64_t l = dp - v;
       val = ooov[l + (int64_t)o * (p + (int64_t)o * (q + (int64_t)o * z))];
     }
-    bv[idx] = val;
+    bv[idx] = (TO)val;
   }
 }
 __global__ void k_accumulate(int64_t n, const double* __restrict__ partials, double* __restrict__ acc) {
@@ -741,9 +765,26 @@ __global__ void k_accumulate(int64_t n, const double* __restrict__ partials, dou
   if (threadIdx.x == 0) *acc += red[0];
 }
 
-static double triples(Ctx& c, bool lowp) {
+// one W-build GEMM of the (T) driver (tag 2), FP64 or FP32 data
+static void tgemm(Ctx& c, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, int64_t sA, const double* B,
+                  int64_t ldb, int64_t sB, double beta, double* W, int64_t ldd, int64_t sD, int64_t batch,
+                  const OutMap* map) {
+  gemm_f64(c.s, false, false, M, N, K, 1.0, A, lda, sA, B, ldb, sB, beta, beta != 0.0 ? W : nullptr, ldd, sD, W, ldd, sD,
+           batch, 2, nullptr, 0, map, false);
+}
+static void tgemm(Ctx& c, int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, int64_t sA, const float* B,
+                  int64_t ldb, int64_t sB, double beta, float* W, int64_t ldd, int64_t sD, int64_t batch,
+                  const OutMap* map) {
+  gemm_f32_ff(c.s, M, N, K, 1.0, A, lda, sA, B, ldb, sB, beta, beta != 0.0 ? W : nullptr, ldd, sD, W, ldd, sD, batch, 2,
+              map);
+}
+
+// (T) with W-build operands and W stored as T (double: FP64 path; float: FP32 MFMA path)
+template <typename T>
+static double triples_t(Ctx& c) {
   prepare(c);
   FH_CHECK(c.have_amp, -6, "no amplitudes for (T)");
+  const bool lowp = std::is_same<T, float>::value;
   const int64_t o = c.o, v = c.v, vv = v * v, nd = v + o;
   const double o_ = (double)o, v_ = (double)v;
   const double ntrip = o_ * (o_ + 1) * (o_ + 2) / 6.0;
@@ -751,27 +792,27 @@ static double triples(Ctx& c, bool lowp) {
   // AV[(x,y),n,d'] and the row-swapped copy AVs[(y,x),n,d'] share one allocation;
   // BVq (Bcat_{p,q} stored [d',z,q,p]) and BVp (stored [d',z,p,q]) likewise
   const int64_t avsz = vv * o * nd, bvsz = nd * v * o * o;
-  Buf AVall = dalloc(c, 2 * avsz), BVqp = dalloc(c, 2 * bvsz);
-  double* AV = AVall.d();
-  double* AVs = AVall.d() + avsz;
-  double* BVq = BVqp.d();
-  double* BVp = BVqp.d() + bvsz;
+  Buf AVall(c.pool.get(), (size_t)(2 * avsz) * sizeof(T)), BVqp(c.pool.get(), (size_t)(2 * bvsz) * sizeof(T));
+  T* AV = static_cast<T*>(AVall.p);
+  T* AVs = AV + avsz;
+  T* BVq = static_cast<T*>(BVqp.p);
+  T* BVp = BVq + bvsz;
   Buf T1p = dalloc(c, v * o), vvoo = dalloc(c, vv * o * o);
   {
     auto g = [](int64_t n) { int64_t b = (n + 255) / 256; return (unsigned)std::min<int64_t>(b, 65536 * 4); };
-    hipLaunchKernelGGL(k_build_av, dim3(g(avsz)), dim3(256), 0, c.s, (int)o, (int)v, c.OVVV.d(), c.T2.d(), AV);
-    hipLaunchKernelGGL(k_build_bv, dim3(g(bvsz)), dim3(256), 0, c.s, (int)o, (int)v, false, c.T2.d(), c.OOOV.d(), BVp);
-    hipLaunchKernelGGL(k_build_bv, dim3(g(bvsz)), dim3(256), 0, c.s, (int)o, (int)v, true, c.T2.d(), c.OOOV.d(), BVq);
+    hipLaunchKernelGGL(k_build_av<T>, dim3(g(avsz)), dim3(256), 0, c.s, (int)o, (int)v, false, c.OVVV.d(), c.T2.d(), AV);
+    hipLaunchKernelGGL(k_build_av<T>, dim3(g(avsz)), dim3(256), 0, c.s, (int)o, (int)v, true, c.OVVV.d(), c.T2.d(), AVs);
+    hipLaunchKernelGGL(k_build_bv<T>, dim3(g(bvsz)), dim3(256), 0, c.s, (int)o, (int)v, false, c.T2.d(), c.OOOV.d(), BVp);
+    hipLaunchKernelGGL(k_build_bv<T>, dim3(g(bvsz)), dim3(256), 0, c.s, (int)o, (int)v, true, c.T2.d(), c.OOOV.d(), BVq);
     FH_HIP(hipGetLastError());
-    perm(c, tv(AV, {v, v, o, nd}), "xynd", tv(AVs, {v, v, o, nd}), "yxnd", 1.0, 0.0);
     perm(c, tv(c.T1.d(), {o, v}), "ia", tv(T1p.d(), {v, o}), "ai", 1.0, 0.0);
     perm(c, tv(c.OVOV.d(), {o, v, o, v}), "iajb", tv(vvoo.d(), {v, v, o, o}), "abij", 1.0, 0.0);
   }
   // k-chunk: W for all k of a chunk, bounded to ~16 GB
   const int64_t v3 = vv * v;
-  int64_t KC = std::max<int64_t>(1, std::min<int64_t>(o, (int64_t)(16e9 / (8.0 * (double)v3))));
-  Buf Wbuf = dalloc(c, v3 * KC);
-  double* W = Wbuf.d();
+  int64_t KC = std::max<int64_t>(1, std::min<int64_t>(o, (int64_t)(16e9 / ((double)sizeof(T) * (double)v3))));
+  Buf Wbuf(c.pool.get(), (size_t)(v3 * KC) * sizeof(T));
+  T* W = static_cast<T*>(Wbuf.p);
   std::vector<int> orb;
   triples_orbits((int)v, orb);
   const int64_t norb = (int64_t)orb.size() / 3;
@@ -798,25 +839,19 @@ static double triples(Ctx& c, bool lowp) {
         // The six permutation classes of ijk2.jl:109-153, G = AV_n (rows (x,y)) * Bcat_{p,q}
         // (cols z), accumulated into W[a,b,c,kk] (W[a,b,c] += G[class-permuted (a,b,c)]).
         // abc: n=j, B_{i,k}, x=a y=b z=c
-        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AV + vv * j, vv * o, 0, BVq + bstride * (k0 + o * i), nd, 0,
-                 0.0, nullptr, vv, 0, W, vv, 0, 1, 2, nullptr, 0, nullptr, lowp);
+        tgemm(c, vv, v * nk, nd, AV + vv * j, vv * o, 0, BVq + bstride * (k0 + o * i), nd, 0, 0.0, W, vv, 0, 1, nullptr);
         // bac: n=i, B_{j,k}, x=b y=a z=c  (row-swapped AV: rows (a,b))
-        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVs + vv * i, vv * o, 0, BVq + bstride * (k0 + o * j), nd, 0,
-                 1.0, W, vv, 0, W, vv, 0, 1, 2, nullptr, 0, nullptr, lowp);
+        tgemm(c, vv, v * nk, nd, AVs + vv * i, vv * o, 0, BVq + bstride * (k0 + o * j), nd, 0, 1.0, W, vv, 0, 1, nullptr);
         // cab: n=i, B_{k,j}, x=c y=a z=b  (rows (a,c), cols (b,kk))
-        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVs + vv * i, vv * o, 0, BVp + bstride * (k0 + o * j), nd, 0,
-                 1.0, W, vv, 0, W, vv, 0, 1, 2, nullptr, 0, &m_cab, lowp);
+        tgemm(c, vv, v * nk, nd, AVs + vv * i, vv * o, 0, BVp + bstride * (k0 + o * j), nd, 0, 1.0, W, vv, 0, 1, &m_cab);
         // cba: n=j, B_{k,i}, x=c y=b z=a  (rows (b,c), cols (a,kk))
-        gemm_f64(c.s, false, false, vv, v * nk, nd, 1.0, AVs + vv * j, vv * o, 0, BVp + bstride * (k0 + o * i), nd, 0,
-                 1.0, W, vv, 0, W, vv, 0, 1, 2, nullptr, 0, &m_cba, lowp);
+        tgemm(c, vv, v * nk, nd, AVs + vv * j, vv * o, 0, BVp + bstride * (k0 + o * i), nd, 0, 1.0, W, vv, 0, 1, &m_cba);
         // acb: n=k, B_{i,j}, x=a y=c z=b  (batched over k: rows (a,c), col b)
-        gemm_f64(c.s, false, false, vv, v, nd, 1.0, AV + vv * k0, vv * o, vv, BVp + bstride * (i + o * j), nd, 0, 1.0, W,
-                 vv, v3, W, vv, v3, nk, 2, nullptr, 0, &m_acb, lowp);
+        tgemm(c, vv, v, nd, AV + vv * k0, vv * o, vv, BVp + bstride * (i + o * j), nd, 0, 1.0, W, vv, v3, nk, &m_acb);
         // bca: n=k, B_{j,i}, x=b y=c z=a  (batched over k: rows (b,c), col a)
-        gemm_f64(c.s, false, false, vv, v, nd, 1.0, AV + vv * k0, vv * o, vv, BVp + bstride * (j + o * i), nd, 0, 1.0, W,
-                 vv, v3, W, vv, v3, nk, 2, nullptr, 0, &m_bca, lowp);
+        tgemm(c, vv, v, nd, AV + vv * k0, vv * o, vv, BVp + bstride * (j + o * i), nd, 0, 1.0, W, vv, v3, nk, &m_bca);
         tg.reset();
-        Timer te(c, "triples_energy", 0.0);
+        Timer te(c, lowp ? "triples_energy_fp32" : "triples_energy", 0.0);
         int64_t nb = 0;
         triples_energy(c.s, (int)o, (int)v, (int)i, (int)j, (int)k0, (int)nk, W, static_cast<const int*>(orbits.p),
                        T1p.d(), vvoo.d(), c.eo.d(), c.ev.d(), part.d(), &nb);
@@ -828,6 +863,8 @@ static double triples(Ctx& c, bool lowp) {
   c.last_et = Et;
   return Et;
 }
+
+static double triples(Ctx& c, bool lowp) { return lowp ? triples_t<float>(c) : triples_t<double>(c); }
 
 }  // namespace fh
 
@@ -903,15 +940,9 @@ void fh_destroy(fh_ctx* x) {
   DeviceGuard g(x->c.device);
   (void)hipStreamSynchronize(x->c.s);
   flush_timers(x->c);
-  {
-    Ctx& c = x->c;
-    c.eo.reset(); c.ev.reset(); c.Boo.reset(); c.Bov.reset(); c.Bvv.reset();
-    c.OOOO.reset(); c.OOOV.reset(); c.OOVV.reset(); c.OVOV.reset(); c.OVVV.reset(); c.Lovov.reset();
-    c.Vperm.reset(); c.Pt.reset(); c.T1.reset(); c.T2.reset(); c.partials.reset(); c.scal.reset();
-  }
-  (void)hipStreamDestroy(x->c.s);
-  x->c.pool.reset();
-  delete x;
+  hipStream_t s = x->c.s;
+  delete x;  // Ctx members go in reverse declaration order: every Buf before the pool
+  (void)hipStreamDestroy(s);
 }
 
 int fh_set_df(fh_ctx* x, int o, int v, int naux, const double* eps_o, const double* eps_v, const double* Boo,
@@ -928,7 +959,7 @@ int fh_set_df(fh_ctx* x, int o, int v, int naux, const double* eps_o, const doub
     c.prepared = false;
     c.have_amp = false;
     c.OOOO.reset(); c.OOOV.reset(); c.OOVV.reset(); c.OVOV.reset(); c.OVVV.reset(); c.Lovov.reset();
-    c.Vperm.reset(); c.Pt.reset(); c.T1.reset(); c.T2.reset();
+    c.Vperm.reset(); c.Vperm32.reset(); c.Pt.reset(); c.T1.reset(); c.T2.reset();
     c.o = o; c.v = v; c.naux = naux;
     const int64_t Q = naux;
     c.eo = dalloc(c, o); c.ev = dalloc(c, v);

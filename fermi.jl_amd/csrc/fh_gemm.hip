@@ -1,15 +1,23 @@
-// FP64 GEMM on CDNA4 matrix cores: D = alpha * op(A) * op(B) + beta * C, column-major.
+// Dense GEMM on CDNA4 matrix cores: D = alpha * op(A) * op(B) + beta * C, column-major.
 //
 // Workgroup: 256 threads = 4 wavefronts in a 2x2 arrangement; each wavefront owns a
-// (BM/2)x(BN/2) block of the output as 16x16 v_mfma_f64_16x16x4f64 accumulators.
-// Operand tiles are staged through LDS with double buffering (global loads for stage
-// s+1 are in flight while stage s is multiplied).  LDS row paddings are chosen so that
-// the 64-bit fragment reads of a half-wavefront hit 32 distinct bank pairs.
+// (BM/2)x(BN/2) block of the output as 16x16 MFMA accumulators.  Operand tiles are staged
+// through LDS with double buffering (global loads for stage s+1 are in flight while stage s
+// is multiplied).  LDS row paddings put the fragment reads of a half (FP64) or full (FP32)
+// wavefront on distinct bank pairs.
 //
-// Tile -> workgroup mapping is XCD aware: consecutive workgroup ids land on different
-// XCDs (round robin over 8), so the linear id is remapped such that each XCD receives a
-// contiguous range of output tiles, M-tile fastest.  Tiles that share a column block of B
-// then run on the same XCD and hit its L2.
+// Precision variants (template F32 and the global element types TI / TO):
+//   F32 = false: FP64 operands, v_mfma_f64_16x16x4f64, FP64 accumulation (TI = TO = double);
+//   F32 = true : operands rounded to FP32 when staged into LDS (TI may be double or float),
+//                v_mfma_f32_16x16x4f32 with FP32 accumulation, alpha/beta epilogue in FP64,
+//                output stored as TO (double or float).
+//
+// Tile -> workgroup mapping: consecutive workgroup ids land on different XCDs (round robin
+// over 8), so the linear id is first remapped such that each XCD receives a contiguous range
+// of tiles.  Within that order tiles are rastered M-fastest (tiles sharing a column block of
+// B are neighbours), or -- when A is the larger operand -- in groups of gn N-tiles whose B
+// blocks fit in L2: each group sweeps all M-tiles, so B stays in L2 and A is streamed once
+// per group instead of once per N-tile.
 #include "fh_internal.h"
 
 #include <algorithm>
@@ -20,18 +28,14 @@ namespace fh {
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// F32 = false: FP64 operands in LDS, v_mfma_f64_16x16x4f64, FP64 accumulation.
-// F32 = true : the FP64 operands are rounded to FP32 while staged into LDS,
-//              v_mfma_f32_16x16x4f32 with FP32 accumulation, alpha/beta epilogue in FP64
-//              (the mixed-precision path's bulk contractions).
-template <int BM, int BN, int BK, bool TA, bool TB, int OCC, int TAG, bool F32>
-__global__ __launch_bounds__(256, OCC) void k_gemm_f64(
-    int64_t M, int64_t N, int64_t K, double alpha, const double* __restrict__ A, int64_t lda, int64_t sA,
-    const double* __restrict__ B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
-    int64_t sC, double* D, int64_t ldd, int64_t sD, int tilesM, int tilesN, int64_t kchunk, int nfast, OutMap om) {
+template <int BM, int BN, int BK, bool TA, bool TB, int OCC, int TAG, bool F32, typename TI, typename TO>
+__global__ __launch_bounds__(256, OCC) void k_gemm(
+    int64_t M, int64_t N, int64_t K, double alpha, const TI* __restrict__ A, int64_t lda, int64_t sA,
+    const TI* __restrict__ B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc, int64_t sC, TO* D,
+    int64_t ldd, int64_t sD, int tilesM, int tilesN, int64_t kchunk, int gn, OutMap om) {
+  static_assert(F32 || std::is_same<TI, double>::value, "FP64 MFMA needs FP64 operands");
   using T = typename std::conditional<F32, float, double>::type;
   using Acc = typename std::conditional<F32, f32x4, f64x4>::type;
-  // paddings: a half (FP64) or full (FP32) wavefront's fragment reads hit distinct banks
   constexpr int PADA = 16, PADB = F32 ? 4 : 2;
   constexpr int WM = BM / 2, WN = BN / 2, MB = WM / 16, NB = WN / 16;
   constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
@@ -41,14 +45,22 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wm = wid & 1, wn = wid >> 1;
 
-  // XCD-aware remap of the linear tile id
+  // XCD-aware remap of the linear tile id, then the raster
   const int64_t nwg = (int64_t)tilesM * tilesN;
   const int64_t w = blockIdx.x;
   const int64_t q = nwg >> 3, r = nwg & 7, x = w & 7;
   const int64_t L = x * q + (x < r ? x : r) + (w >> 3);
-  // raster: M-tile fastest (tiles sharing a column block of B are neighbours) or, when A is
-  // the larger operand, N-tile fastest (a row block of A is reused by all N-tiles at once)
-  const int64_t mt = nfast ? L / tilesN : L % tilesM, nt = nfast ? L % tilesN : L / tilesM;
+  int64_t mt, nt;
+  if (gn <= 0) {
+    mt = L % tilesM;
+    nt = L / tilesM;
+  } else {
+    const int64_t g = L / ((int64_t)tilesM * gn);
+    const int64_t rem = L - g * (int64_t)tilesM * gn;
+    const int64_t wdt = (tilesN - g * gn < gn) ? tilesN - g * gn : gn;
+    mt = rem / wdt;
+    nt = g * gn + rem % wdt;
+  }
   const int64_t m0 = mt * BM, n0 = nt * BN;
 
   const int64_t bz = blockIdx.y;
@@ -65,7 +77,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
   if (C) C += bz * sC;
   D += bz * sD;
 
-  double ra[LA], rb[LB];
+  TI ra[LA], rb[LB];
   auto gload = [&](int64_t k0) {
 #pragma unroll
     for (int t = 0; t < LA; ++t) {
@@ -73,7 +85,7 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
       int mm, kk;
       if (!TA) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
       const int64_t gm = m0 + mm, gk = k0 + kk;
-      double val = 0.0;
+      TI val = 0;
       if (gm < M && gk < K) val = TA ? A[gk + lda * gm] : A[gm + lda * gk];
       ra[t] = val;
     }
@@ -82,9 +94,9 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
       const int e = tid + 256 * t;
       int nn, kk;
       if (!TB) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
-      const int64_t gn = n0 + nn, gk = k0 + kk;
-      double val = 0.0;
-      if (gn < N && gk < K) val = TB ? B[gn + ldb * gk] : B[gk + ldb * gn];
+      const int64_t gcol = n0 + nn, gk = k0 + kk;
+      TI val = 0;
+      if (gcol < N && gk < K) val = TB ? B[gcol + ldb * gk] : B[gk + ldb * gcol];
       rb[t] = val;
     }
   };
@@ -138,10 +150,9 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
   }
 
   // epilogue: lane holds column lane&15 and rows (lane>>4)+4r (FP64 MFMA) or 4(lane>>4)+r
-  // (FP32 MFMA) of each 16x16 block.  Output
-  // address: column-major with ldd, or (om.rmod > 0) the two-mode map
-  // row r -> (r % rmod)*rs0 + (r / rmod)*rs1, column c -> (c % cmod)*cs0 + (c / cmod)*cs1
-  // (cmod = 0: c*cs0), which lets a GEMM accumulate into a permuted 3-/4-index tensor.
+  // (FP32 MFMA) of each 16x16 block.  Output address: column-major with ldd, or (om.rmod > 0)
+  // the two-mode map row r -> (r % rmod)*rs0 + (r / rmod)*rs1, column c -> (c % cmod)*cs0 +
+  // (c / cmod)*cs1 (cmod = 0: c*cs0), which lets a GEMM accumulate into a permuted tensor.
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int64_t col = n0 + wn * WN + j * 16 + (lane & 15);
@@ -157,38 +168,49 @@ __global__ __launch_bounds__(256, OCC) void k_gemm_f64(
         if (row < M) {
           const int64_t roff = om.rmod > 0 ? (row % om.rmod) * om.rs0 + (row / om.rmod) * om.rs1 : row;
           double val = alpha * (double)acc[i][j][rr];
-          if (beta != 0.0) val += beta * (om.rmod > 0 ? C[roff + coff] : C[row + ldc * col]);
-          D[roff + coff] = val;
+          if (beta != 0.0) val += beta * (double)(om.rmod > 0 ? C[roff + coff] : C[row + ldc * col]);
+          D[roff + coff] = (TO)val;
         }
       }
   }
 }
 
-template <int BM, int BN, int BK, bool TA, bool TB, int TAG, bool F32>
-static void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
-                   int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
-                   int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om) {
+// L2 budget of the B blocks of one raster group (bytes)
+static constexpr int64_t kGroupBytes = 1 << 20;
+
+template <int BM, int BN, int BK, bool TA, bool TB, int TAG, bool F32, typename TI, typename TO>
+static void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda, int64_t sA,
+                   const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc, int64_t sC, TO* D,
+                   int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om) {
   const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   FH_CHECK(tm * tn < (int64_t)1 << 31 && batch < 65536, -3, "gemm grid too large");
   dim3 grid((unsigned)(tm * tn), (unsigned)batch);
-  const int nfast = (M > N) ? 1 : 0;   // stream the larger of A (M x K) and B (K x N) once
-  hipLaunchKernelGGL((k_gemm_f64<BM, BN, BK, TA, TB, (BM * BN > 8192 ? 1 : 4), TAG, F32>), grid, dim3(256), 0, s, M, N, K, alpha, A,
-                     lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, nfast, om);
+  // A larger than B: group N-tiles so that a group's B blocks stay in L2
+  int gn = 0;
+  if (M > N) {
+    const int64_t kl = kchunk > 0 ? kchunk : K;
+    const int64_t per = std::max<int64_t>(1, kl * BN * (int64_t)sizeof(TI));
+    gn = (int)std::max<int64_t>(1, std::min<int64_t>(tn, kGroupBytes / per));
+  }
+  hipLaunchKernelGGL((k_gemm<BM, BN, BK, TA, TB, (BM * BN > 8192 ? 1 : 4), TAG, F32, TI, TO>), grid, dim3(256), 0, s, M,
+                     N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, gn, om);
   FH_HIP(hipGetLastError());
 }
 
-template <bool TA, bool TB, int TAG, bool F32>
-static void pick(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
-                 int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C, int64_t ldc,
-                 int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om) {
-  // 64x64 tiles at up to four workgroups per CU measured fastest on every shape of this
-  // engine (the latency of the next operand tile is hidden by the other workgroups);
-  // BK = 8 when the per-tile K loop is short.
+// 64x64 tiles at up to four workgroups per CU measured fastest on every shape of this engine
+// (the latency of the next operand tile is hidden by the other workgroups); BK = 8 when the
+// per-tile K loop is short.
+template <bool TA, bool TB, int TAG, bool F32, typename TI, typename TO>
+static void pick(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda, int64_t sA,
+                 const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc, int64_t sC, TO* D,
+                 int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om) {
   const int64_t kl = kchunk > 0 ? kchunk : K;
   if (kl <= 512)
-    launch<64, 64, 8, TA, TB, TAG, F32>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk, om);
+    launch<64, 64, 8, TA, TB, TAG, F32, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD,
+                                                batch, kchunk, om);
   else
-    launch<64, 64, 16, TA, TB, TAG, F32>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, kchunk, om);
+    launch<64, 64, 16, TA, TB, TAG, F32, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd,
+                                                 sD, batch, kchunk, om);
 }
 
 // split-K sizing: few output tiles and a long K leave most CUs idle; cut K into slices
@@ -219,17 +241,22 @@ __global__ void k_splitk_reduce(int64_t M, int64_t N, int64_t S, const double* _
   }
 }
 
+static void check_args(int64_t M, int64_t K, int64_t N, bool ta, bool tb, int64_t lda, int64_t ldb, int64_t ldd,
+                       double beta, const void* C, const void* D, int64_t ldc, const OutMap* map) {
+  FH_CHECK(K >= 0, -3, "gemm: negative K");
+  FH_CHECK(lda >= (ta ? (K > 0 ? K : 1) : M) && ldb >= (tb ? N : (K > 0 ? K : 1)) && (map || ldd >= M), -3,
+           "gemm: leading dimension too small");
+  FH_CHECK(beta == 0.0 || (C != nullptr && (map || ldc >= M)), -3, "gemm: C missing");
+  FH_CHECK(!map || (map->rmod > 0 && (beta == 0.0 || C == D)), -3, "gemm: mapped output needs rmod > 0 and C == D");
+}
+
 void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
               int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C,
               int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int tag, double* work,
               int64_t work_elems, const OutMap* map, bool f32) {
-  const OutMap om = map ? *map : OutMap{};
-  FH_CHECK(!map || (map->rmod > 0 && (beta == 0.0 || C == D)), -3, "gemm: mapped output needs rmod > 0 and C == D");
   if (M <= 0 || N <= 0 || batch <= 0) return;
-  FH_CHECK(K >= 0, -3, "gemm: negative K");
-  FH_CHECK(lda >= (ta ? (K > 0 ? K : 1) : M) && ldb >= (tb ? N : (K > 0 ? K : 1)) && ldd >= M, -3,
-           "gemm: leading dimension too small");
-  FH_CHECK(beta == 0.0 || (C != nullptr && ldc >= M), -3, "gemm: C missing");
+  check_args(M, K, N, ta, tb, lda, ldb, ldd, beta, C, D, ldc, map);
+  const OutMap om = map ? *map : OutMap{};
   const int64_t sl = map ? 1 : splitk_slices(M, N, K, batch);
   const bool split = sl > 1 && work != nullptr && work_elems >= sl * M * N;
   const int64_t kchunk = split ? (K + sl - 1) / sl : 0;
@@ -238,16 +265,22 @@ void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, 
   const double* Cc = split ? nullptr : C;
   double* Dd = split ? work : D;
   const int64_t lddd = split ? M : ldd, sDD = split ? M * N : sD;
-#define FH_PICK2(TA_, TB_, F_)                                                                                  \
+#define FH_PICK2(TA_, TB_, F_)                                                                                      \
   do {                                                                                                              \
-    if (tag == 1) pick<TA_, TB_, 1, F_>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om); \
-    else if (tag == 2) pick<TA_, TB_, 2, F_>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om); \
-    else pick<TA_, TB_, 0, F_>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, kchunk, om);         \
+    if (tag == 1)                                                                                                   \
+      pick<TA_, TB_, 1, F_, double, double>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, \
+                                            kchunk, om);                                                            \
+    else if (tag == 2)                                                                                              \
+      pick<TA_, TB_, 2, F_, double, double>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, \
+                                            kchunk, om);                                                            \
+    else                                                                                                            \
+      pick<TA_, TB_, 0, F_, double, double>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, \
+                                            kchunk, om);                                                            \
   } while (0)
-#define FH_PICK(TA_, TB_)                   \
-  do {                                      \
-    if (f32) FH_PICK2(TA_, TB_, true);      \
-    else FH_PICK2(TA_, TB_, false);         \
+#define FH_PICK(TA_, TB_)              \
+  do {                                 \
+    if (f32) FH_PICK2(TA_, TB_, true); \
+    else FH_PICK2(TA_, TB_, false);    \
   } while (0)
   if (ta) {
     if (tb) FH_PICK(true, true); else FH_PICK(true, false);
@@ -262,6 +295,36 @@ void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, 
     hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)g), dim3(256), 0, s, M, N, nb, work, alpha, beta, C, ldc, D, ldd);
     FH_HIP(hipGetLastError());
   }
+}
+
+// FP32-MFMA GEMMs with FP32 operands and/or outputs in HBM (NN, no split-K)
+template <int TAG, typename TI, typename TO>
+static void gemm_nn_f32(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda,
+                        int64_t sA, const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc,
+                        int64_t sC, TO* D, int64_t ldd, int64_t sD, int64_t batch, const OutMap* map) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  check_args(M, K, N, false, false, lda, ldb, ldd, beta, C, D, ldc, map);
+  const OutMap om = map ? *map : OutMap{};
+  pick<false, false, TAG, true, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch,
+                                        0, om);
+}
+
+void gemm_f32_fd(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const float* A, int64_t lda,
+                 const float* B, int64_t ldb, double beta, const double* C, int64_t ldc, double* D, int64_t ldd, int tag) {
+  if (tag == 1) gemm_nn_f32<1>(s, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, D, ldd, 0, 1, nullptr);
+  else gemm_nn_f32<0>(s, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, D, ldd, 0, 1, nullptr);
+}
+
+void gemm_f32_df(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                 const double* B, int64_t ldb, double beta, const float* C, int64_t ldc, float* D, int64_t ldd) {
+  gemm_nn_f32<0>(s, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, D, ldd, 0, 1, nullptr);
+}
+
+void gemm_f32_ff(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const float* A, int64_t lda, int64_t sA,
+                 const float* B, int64_t ldb, int64_t sB, double beta, const float* C, int64_t ldc, int64_t sC, float* D,
+                 int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map) {
+  if (tag == 2) gemm_nn_f32<2>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, map);
+  else gemm_nn_f32<0>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, map);
 }
 
 }  // namespace fh

@@ -95,6 +95,16 @@ void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, 
               int64_t batch = 1, int tag = 0, double* work = nullptr, int64_t work_elems = 0,
               const OutMap* map = nullptr, bool f32 = false);
 // f32 = true: operands rounded to FP32 in LDS, FP32 MFMA with FP32 accumulation (mixed precision)
+
+// FP32-MFMA GEMMs (NN) with FP32 data in HBM: _fd float operands -> double output,
+// _df double operands -> float output, _ff float operands -> float output (strided batch, map)
+void gemm_f32_fd(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const float* A, int64_t lda,
+                 const float* B, int64_t ldb, double beta, const double* C, int64_t ldc, double* D, int64_t ldd, int tag);
+void gemm_f32_df(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                 const double* B, int64_t ldb, double beta, const float* C, int64_t ldc, float* D, int64_t ldd);
+void gemm_f32_ff(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const float* A, int64_t lda, int64_t sA,
+                 const float* B, int64_t ldb, int64_t sB, double beta, const float* C, int64_t ldc, int64_t sC, float* D,
+                 int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map);
 // workspace (doubles) that gemm_f64 would use to split K for this shape (0: no split)
 int64_t gemm_splitk_elems(int64_t M, int64_t N, int64_t K, int64_t batch);
 // tag selects a distinct kernel symbol for profiling: 0 generic, 1 PP ladder, 2 (T) W build
@@ -129,7 +139,8 @@ void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, 
 // orbits: device list of cube-block triples (A>=B>=C), 3 ints each (triples_orbits)
 int triples_orbit_count(int v);
 void triples_orbits(int v, std::vector<int>& out);
-void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const double* W, const int* orbits,
+template <typename TW>  // W stored as double (FP64 path) or float (FP32 W build)
+void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TW* W, const int* orbits,
                     const double* t1p, const double* vvoo, const double* eo, const double* ev, double* partials,
                     int64_t* nblocks);
 

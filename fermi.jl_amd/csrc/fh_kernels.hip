@@ -333,7 +333,8 @@ void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, 
 // every element of W is read once per triple.
 static constexpr int CT = 8, CT3 = CT * CT * CT;
 
-__global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int j, int k0, const double* __restrict__ W,
+template <typename TW>
+__global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int j, int k0, const TW* __restrict__ W,
                                                         const int* __restrict__ orbits,
                                                         const double* __restrict__ t1p, const double* __restrict__ vvoo,
                                                         const double* __restrict__ eo, const double* __restrict__ ev,
@@ -358,7 +359,7 @@ __global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int
   double acc = 0.0;
   if (wocc != 0.0) {
     const int64_t vv = (int64_t)v * v;
-    const double* Wk = W + vv * v * kk;
+    const TW* Wk = W + vv * v * kk;
     const double* vjk = vvoo + vv * (j + (int64_t)o * k);
     const double* vik = vvoo + vv * (i + (int64_t)o * k);
     const double* vij = vvoo + vv * (i + (int64_t)o * j);
@@ -370,7 +371,7 @@ __global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int
         const int a = P0 + (e & 7), b = Q0 + ((e >> 3) & 7), c = R0 + (e >> 6);
         double w = 0.0, val = 0.0;
         if (a < v && b < v && c < v) {
-          w = Wk[a + (int64_t)v * b + vv * c];
+          w = (double)Wk[a + (int64_t)v * b + vv * c];
           val = w + t1p[a + (int64_t)v * i] * vjk[b + (int64_t)v * c] + vik[a + (int64_t)v * c] * t1p[b + (int64_t)v * j] +
                 vij[a + (int64_t)v * b] * t1p[c + (int64_t)v * k];
         }
@@ -426,14 +427,19 @@ void triples_orbits(int v, std::vector<int>& out) {
       for (int c = 0; c <= b; ++c) { out.push_back(a); out.push_back(b); out.push_back(c); }
 }
 
-void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const double* W, const int* orbits,
+template <typename TW>
+void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TW* W, const int* orbits,
                     const double* t1p, const double* vvoo, const double* eo, const double* ev, double* partials,
                     int64_t* nblocks) {
   const int norb = triples_orbit_count(v);
   *nblocks = (int64_t)norb * nk;
-  hipLaunchKernelGGL(k_triples_energy, dim3((unsigned)norb, (unsigned)nk), dim3(256), 0, s, o, v, i, j, k0, W, orbits,
-                     t1p, vvoo, eo, ev, partials);
+  hipLaunchKernelGGL(k_triples_energy<TW>, dim3((unsigned)norb, (unsigned)nk), dim3(256), 0, s, o, v, i, j, k0, W,
+                     orbits, t1p, vvoo, eo, ev, partials);
   FH_HIP(hipGetLastError());
 }
+template void triples_energy<double>(hipStream_t, int, int, int, int, int, int, const double*, const int*, const double*,
+                                     const double*, const double*, const double*, double*, int64_t*);
+template void triples_energy<float>(hipStream_t, int, int, int, int, int, int, const float*, const int*, const double*,
+                                    const double*, const double*, const double*, double*, int64_t*);
 
 }  // namespace fh
