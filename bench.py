@@ -22,6 +22,10 @@ for _p in (ROOT, os.path.join(ROOT, "fermi.jl_amd")):
 FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
+PREC_NAMES = {"double": "FP64", "mixed": "mixed FP32/FP64",
+              "bf16x3": "mixed split-BF16x3/FP32/FP64"}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -34,7 +38,7 @@ def phase_diff(after, before):
     return out
 
 
-def cpu_baseline(o, v, Q, seed, niter, budget_s):
+def cpu_baseline(work, niter, budget_s):
     """CPU oracle (numpy restatement of Fermi's RCCSDa / ijk.jl) on a bounded sample of the same
     workload, scaled to a time-to-solution estimate
         t = t_integrals + niter * (t_ladder + t_rest) + t_(T):
@@ -51,7 +55,8 @@ def cpu_baseline(o, v, Q, seed, niter, budget_s):
     from fermihip import synth
     from oracle import rccsd as R
     from oracle import triples as TT
-    eo, ev, Boo, Bov, Bvv = synth.molecular_like(o, v, Q, seed)
+    eo, ev, Boo, Bov, Bvv, _, _ = work
+    o, v, Q = Bov.shape[1], Bov.shape[2], Bov.shape[0]
     t0 = time.perf_counter()
     m = R.build_moints(Boo, Bov, Bvv, with_vvvv=False)
     t_int = time.perf_counter() - t0
@@ -79,7 +84,7 @@ def cpu_baseline(o, v, Q, seed, niter, budget_s):
     if o * v <= so * sv:
         so, sv, sq, ms, e1, e2 = o, v, Q, m, T1, T2
     else:
-        a_, b_, c_, d_, f_ = synth.molecular_like(so, sv, sq, seed)
+        a_, b_, c_, d_, f_, _, _ = synth.water_cluster_like(synth.WATERS["water4"], 0)
         ms = R.build_moints(c_, d_, f_, with_vvvv=False)
         e1, e2 = R.mp2_guess(ms, a_, b_)
     ms_l = dict(ms)
@@ -108,12 +113,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="water10", help="water1 | water4 | water10 (synthetic DF shapes)")
+    ap.add_argument("--config", default="water10", help="water1 | water4 | water10 (synthetic (H2O)n clusters)")
+    ap.add_argument("--guess", default="auto", choices=["auto", "learned", "mp2"],
+                    help="initial T2: learned (the committed network; auto = learned when the model file exists) "
+                         "or mp2")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--diis-prec", default="single", choices=["single", "double"])
-    ap.add_argument("--precision", default="mixed", choices=["double", "mixed"],
-                    help="mixed: FP32-MFMA contractions + FP64 polish and FP32 (T) W build; its energies are "
-                         "checked against an FP64 run of the same workload (reported as error_vs_fp64)")
+    ap.add_argument("--precision", default="mixed", choices=["double", "mixed", "bf16x3"],
+                    help="mixed: FP32-MFMA contractions + FP64 polish and FP32 (T) W build; bf16x3: split-BF16x3 "
+                         "contractions, then FP32, then FP64 polish, and a split-BF16x3 (T) W build.  Reduced-"
+                         "precision runs are checked against an FP64 run of the same workload (error_vs_fp64)")
+    ap.add_argument("--polish-iters", type=int, default=-1,
+                    help="FP64 polish iterations after the reduced-precision phases (-1: to convergence)")
+    ap.add_argument("--verbose", action="store_true", help="print the CC iteration table")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--phase", default="full", choices=["full", "triples", "copy"],
@@ -141,19 +153,30 @@ def main():
     import fermihip
     from fermihip import synth
 
-    o, v, Q = synth.SHAPES[args.config]
-    log(f"bench: config {args.config} o={o} v={v} naux={Q} seed={args.seed}")
-    eo, ev, Boo, Bov, Bvv = synth.molecular_like(o, v, Q, args.seed)
+    from fermihip import train_guess
+    nw = synth.WATERS[args.config]
+    work = synth.water_cluster_like(nw, args.seed)
+    eo, ev, Boo, Bov, Bvv, sites_o, pos = work
+    o, v, Q = Bov.shape[1], Bov.shape[2], Bov.shape[0]
+    log(f"bench: config {args.config} ((H2O){nw}) o={o} v={v} naux={Q} seed={args.seed}")
+    model = train_guess.load_model() if args.guess != "mp2" else None
+    if args.guess == "learned" and model is None:
+        raise SystemExit("bench.py: --guess learned but no model file (fermi.jl_amd/fermihip/t2_guess_water.npz)")
+    guess_kind = 1 if model is not None else 0
+    pair_d = synth.pair_distances(sites_o, pos)
     ctx = fermihip.Context(0)
-    opts = fermihip.Options(diis_prec=args.diis_prec, precision=args.precision)
-    tprec = 1 if args.precision == "mixed" else 0
+    opts = fermihip.Options(diis_prec=args.diis_prec, precision=args.precision, polish_iters=args.polish_iters,
+                            verbose=args.verbose)
+    tprec = fermihip.PRECISIONS[args.precision]
 
     def step():
         t0 = time.perf_counter()
         ctx.set_df(eo, ev, Boo, Bov, Bvv)      # host -> HBM (untimed part of the step)
+        if guess_kind == 1:
+            ctx.set_t2_model(model, pair_d)    # network + pair distances (untimed, like the upload)
         t1 = time.perf_counter()
         ctx.prepare()
-        ctx.guess(copy_out=False)
+        ctx.guess(kind=guess_kind, copy_out=False)
         if args.phase == "triples":
             r = None
         else:
@@ -194,35 +217,46 @@ def main():
                 "calls": p["calls"] // max(1, args.steps), "flops": p["flops"] / max(1, args.steps)}
 
     kern = {n: rate(n) for n in ph}
-    # dominant MFMA kernel of the step: the PP ladder GEMM or the (T) W-build GEMM (distinct
-    # kernel symbols via the tag template argument, so rocprofv3 lists them separately)
+    # dominant MFMA kernel of the step: the PP ladder GEMM or the (T) W-build GEMM at whichever
+    # precision took the most time (distinct kernel symbols via the tag template argument, so
+    # rocprofv3 lists them separately)
     FP32_MFMA_PEAK_TFLOPS = 157.3
-    sfx = "_fp32" if args.precision == "mixed" else ""
-    kt = "float" if sfx else "double"
+    BF16_MFMA_PEAK_TFLOPS = 2500.0
     launches_per_call = {"ladder": 1, "triples_gemm": 6}
-    f32 = "true" if sfx else "false"
-    ti = "float" if sfx else "double"
-    # template arguments: BM, BN, BK, TA, TB, waves-per-CU hint, tag, FP32 MFMA, operand type, output type
-    names = {"ladder": f"k_gemm<64, 64, 16, false, false, 4, 1, {f32}, {ti}, double> (PP ladder, tag 1, {kt} MFMA)",
-             "triples_gemm": f"k_gemm<64, 64, 8, false, false, 4, 2, {f32}, {ti}, {ti}> ((T) W build, tag 2, {kt} MFMA)"}
-    peak = FP32_MFMA_PEAK_TFLOPS if sfx else FP64_MFMA_PEAK_TFLOPS
+    # FP64/FP32 kernel template arguments: BM, BN, BK, TA, TB, waves-per-EU hint, tag, FP32 MFMA,
+    # operand type, output type; bf16x3 kernel: TA, TB, tag, operand type, output type, vector loads
+    variants = {
+        "": ({"ladder": "k_gemm<64, 64, 16, false, false, 4, 1, false, double, double> (PP ladder, FP64 MFMA)",
+              "triples_gemm": "k_gemm<64, 64, 8, false, false, 4, 2, false, double, double> ((T) W build, FP64 MFMA)"},
+             FP64_MFMA_PEAK_TFLOPS, "FP64 MFMA dense"),
+        "_fp32": ({"ladder": "k_gemm<64, 64, 16, false, false, 4, 1, true, float, double> (PP ladder, FP32 MFMA)",
+                   "triples_gemm": "k_gemm<64, 64, 8, false, false, 4, 2, true, float, float> ((T) W build, FP32 MFMA)"},
+                  FP32_MFMA_PEAK_TFLOPS, "FP32 MFMA dense"),
+        "_bf16x3": ({"ladder": "k_gemm_bf3<false, false, 1, float, double, true> (PP ladder, split-BF16x3 MFMA)",
+                     "triples_gemm": "k_gemm_bf3<false, false, 2, float, float, true> ((T) W build, split-BF16x3 MFMA)"},
+                    BF16_MFMA_PEAK_TFLOPS / 3.0, "BF16 MFMA dense / 3 products (FP32-equivalent)"),
+    }
 
-    def roof_of(n):
+    def roof_of(n, sfx):
         kd = kern.get(n + sfx)
         if not kd:
             return None
+        names, peak, pname = variants[sfx]
         nl = kd["calls"] * launches_per_call[n]
-        return {"kernel": names[n], "bound": "mfma", "achieved": kd["tflops"], "peak": peak, "unit": "TFLOP/s",
-                "frac": kd["tflops"] / peak, "traffic": None, "flops_per_launch": kd["flops"] / max(1, nl),
-                "avg_launch_us": kd["ms"] * 1e3 / max(1, nl), "launches": nl}
+        return {"kernel": names[n], "bound": "mfma", "achieved": kd["tflops"], "peak": peak, "peak_kind": pname,
+                "unit": "TFLOP/s", "frac": kd["tflops"] / peak, "traffic": None,
+                "flops_per_launch": kd["flops"] / max(1, nl), "avg_launch_us": kd["ms"] * 1e3 / max(1, nl),
+                "launches": nl}
 
-    cand = [n for n in ("ladder", "triples_gemm") if kern.get(n + sfx)]
-    dom = max(cand, key=lambda n: kern[n + sfx]["ms"])
-    roof = roof_of(dom)
-    roof_other = roof_of("ladder" if dom == "triples_gemm" else "triples_gemm")
+    cand = [(n, sfx) for n in ("ladder", "triples_gemm") for sfx in variants if kern.get(n + sfx)]
+    dom = max(cand, key=lambda t: kern[t[0] + t[1]]["ms"])
+    roof = roof_of(*dom)
+    others = [t for t in cand if t[0] != dom[0]]
+    roof_other = roof_of(*max(others, key=lambda t: kern[t[0] + t[1]]["ms"])) if others else None
     fp64_ref = None
-    if args.precision == "mixed":
-        # the FP64 GPU path on the same workload: reference energies and its time-to-solution
+    if args.precision != "double":
+        # the plain FP64 GPU path (MP2 guess) on the same workload: reference energies and its
+        # time-to-solution
         opts64 = fermihip.Options(diis_prec=args.diis_prec, precision="double")
         ctx.set_df(eo, ev, Boo, Bov, Bvv)
         t1 = time.perf_counter()
@@ -233,20 +267,22 @@ def main():
         t64 = time.perf_counter() - t1
         log(f"bench: FP64 path {t64:.3f} s niter={r64.niter} Ecorr={r64.correlation:.12f} Et={Et64:.12e}")
         fp64_ref = {"time_to_solution_s": t64, "ccsd_correlation": r64.correlation, "triples": Et64,
-                    "niter": r64.niter, "speedup_mixed_vs_fp64": t64 / tts,
+                    "niter": r64.niter, "guess": "mp2", "speedup_vs_fp64": t64 / tts,
                     "error_vs_fp64_Eh": {"ccsd": r.correlation - r64.correlation, "triples": Et - Et64,
                                          "total": (r.correlation + Et) - (r64.correlation + Et64)}}
     out = {
         "metric": "DF-RCCSD(T) time-to-solution (s) + MFMA TFLOP/s at <=1 uEh vs FP64 GPU path",
         "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": tts * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "double" else "f32/f64", "data": "synthetic (seeded DF tensors of the cluster's shape)",
-        "config": {"workload": f"DF-RCCSD(T) {'mixed FP32/FP64' if args.precision == 'mixed' else 'FP64'}, {args.config} shape: o={o} v={v} naux={Q} (synthetic "
-                               f"molecular-like DF tensors, seed {args.seed})",
+        "dtype": {"double": "f64", "mixed": "f32/f64", "bf16x3": "bf16x3/f32/f64"}[args.precision], "data": "synthetic (seeded DF tensors of the cluster's shape)",
+        "config": {"workload": f"DF-RCCSD(T) {PREC_NAMES[args.precision]}, {args.config} shape: o={o} v={v} naux={Q} (synthetic "
+                               f"(H2O){nw} lattice-cluster DF tensors, seed {args.seed}), "
+                               f"{'learned' if guess_kind else 'MP2'} T2 guess",
                    "o": o, "v": v, "naux": Q, "precision": args.precision, "diis_prec": args.diis_prec,
+                   "polish_iters": args.polish_iters, "guess": "learned" if guess_kind else "mp2",
                    "parallelism": "single GPU"},
         "energies": {"ccsd_correlation": r.correlation, "triples": Et, "niter": r.niter, "converged": bool(r.converged),
-                     "niter_fp32": r.niter_mixed},
+                     "niter_reduced_precision": r.niter_mixed, "niter_bf16x3": r.niter_bf3},
         "fp64_path": fp64_ref,
         "pcie_inclusive_s": tts + sum(uploads) / len(uploads),
         "roofline": roof,
@@ -255,7 +291,7 @@ def main():
     }
     if args.cpu_baseline == "auto":
         log("bench: timing CPU oracle sample")
-        out["cpu_baseline"] = cpu_baseline(o, v, Q, args.seed, r.niter, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(work, r.niter, args.cpu_seconds)
     print(json.dumps(out), flush=True)
     ctx.close()
 

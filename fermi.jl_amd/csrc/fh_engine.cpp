@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <random>
 #include <sstream>
 #include <type_traits>
 
@@ -101,7 +102,7 @@ struct Ctx {
   std::unique_ptr<Pool> pool;
   int o = 0, v = 0, naux = 0;
   bool have_df = false, prepared = false;
-  bool lowp = false;  // residual contractions as FP32 MFMA (mixed-precision phase)
+  int lowp = 0;  // residual contractions: 0 FP64 MFMA, 1 FP32 MFMA, 2 split-BF16x3 MFMA (large GEMMs)
   Buf eo, ev, Boo, Bov, Bvv;
   // 4-index blocks (Julia index order, column-major)
   Buf OOOO, OOOV, OOVV, OVOV, OVVV, Lovov, Vperm, Pt;
@@ -111,6 +112,12 @@ struct Ctx {
   bool have_amp = false;
   // scratch for reductions
   Buf partials, scal;
+  // learned T2 guess: network parameters (FP32), feature mean / inverse scale, occupied pair
+  // distances [o,o]; attached per molecule (fh_set_df detaches it)
+  Buf mlp_P, mlp_xm, mlp_xs, pair_d;
+  int mlp_h = 0;
+  double mlp_ys = 0.0;
+  bool have_model = false;
   std::map<std::string, Phase> phases;
   std::vector<Pending> pending;
   std::vector<IterRow> iters;
@@ -298,17 +305,23 @@ static void contract(Ctx& c, double alpha, const TView& A, const std::string& ia
   const int64_t wsz = gemm_splitk_elems(Msz, Nsz, Ksz, 1);
   Buf work;
   if (wsz) work = dalloc(c, wsz);
+  // split-BF16x3 for the o^3 v^3-class GEMMs of the mixed phase; smaller ones (and the split-K
+  // shapes) stay on FP32 MFMA
+  const bool bf3 = c.lowp == 2 && wsz == 0 && (double)Msz * (double)Nsz * (double)Ksz >= 1e9;
+  auto gemm = [&](bool ta, bool tb, int64_t m, int64_t n, double al, const double* a, int64_t la, const double* b,
+                  int64_t lb, double be, const double* cc, double* d, int64_t ld) {
+    if (bf3) gemm_bf3<double, double>(c.s, ta, tb, m, n, Ksz, al, a, la, 0, b, lb, 0, be, cc, ld, 0, d, ld, 0, 1, 0, nullptr);
+    else gemm_f64(c.s, ta, tb, m, n, Ksz, al, a, la, 0, b, lb, 0, be, cc, ld, 0, d, ld, 0, 1, 0, work.d(), wsz, nullptr,
+                  c.lowp != 0);
+  };
   if (cmode == 0) {
-    gemm_f64(c.s, transA, transB, Msz, Nsz, Ksz, alpha, pA, lda, 0, pB, ldb, 0, beta, C.p, Msz, 0, C.p, Msz, 0, 1, 0,
-             work.d(), wsz, nullptr, c.lowp);
+    gemm(transA, transB, Msz, Nsz, alpha, pA, lda, pB, ldb, beta, C.p, C.p, Msz);
   } else if (cmode == 1) {
     // C^T (Nsz x Msz) = op(B)^T op(A)^T
-    gemm_f64(c.s, !transB, !transA, Nsz, Msz, Ksz, alpha, pB, ldb, 0, pA, lda, 0, beta, C.p, Nsz, 0, C.p, Nsz, 0, 1, 0,
-             work.d(), wsz, nullptr, c.lowp);
+    gemm(!transB, !transA, Nsz, Msz, alpha, pB, ldb, pA, lda, beta, C.p, C.p, Nsz);
   } else {
     tC = dalloc(c, Msz * Nsz);
-    gemm_f64(c.s, transA, transB, Msz, Nsz, Ksz, 1.0, pA, lda, 0, pB, ldb, 0, 0.0, nullptr, Msz, 0, tC.d(), Msz, 0, 1,
-             0, work.d(), wsz, nullptr, c.lowp);
+    gemm(transA, transB, Msz, Nsz, 1.0, pA, lda, pB, ldb, 0.0, nullptr, tC.d(), Msz);
     TView T;
     T.p = tC.d();
     T.nd = (int)(M.size() + N.size());
@@ -332,8 +345,8 @@ static void require_df(Ctx& c) { FH_CHECK(c.have_df, -6, "no DF tensors: call fh
 static void prepare(Ctx& c) {
   require_df(c);
   if (c.prepared) return;
-  const bool keep = c.lowp;
-  c.lowp = false;  // integrals are assembled in FP64
+  const int keep = c.lowp;
+  c.lowp = 0;  // integrals are assembled in FP64
   const int64_t o = c.o, v = c.v, Q = c.naux;
   Timer t(c, "integrals", 2.0 * Q * (double)(o * o * o * o + o * o * o * v + o * o * v * v + o * v * o * v +
                                               o * v * v * v + v * v * v * v));
@@ -466,14 +479,20 @@ static void residual(Ctx& c, const double* t1p, const double* t2p, double* R1, d
         Timer tl(c, "ladder_build_fp32", 2.0 * o_ * v_ * v_ * v_ * v_);
         gemm_f32_df(c.s, v3, v, o, -1.0, c.Pt.d(), v3, t1p, o, 0.5, c.Vperm32.f(), v3, bY.f(), v3);
       }
-      Timer tl(c, "ladder_fp32", 2.0 * o_ * o_ * v_ * v_ * v_ * v_);
-      gemm_f32_fd(c.s, o * o, v * v, v * v, 1.0, tau32.f(), o * o, bY.f(), v * v, 1.0, P.p, o * o, P.p, o * o, 1);
+      if (c.lowp == 2) {
+        Timer tl(c, "ladder_bf16x3", 2.0 * o_ * o_ * v_ * v_ * v_ * v_);
+        gemm_bf3<float, double>(c.s, false, false, o * o, v * v, v * v, 1.0, tau32.f(), o * o, 0, bY.f(), v * v, 0, 1.0,
+                                P.p, o * o, 0, P.p, o * o, 0, 1, 1, nullptr);
+      } else {
+        Timer tl(c, "ladder_fp32", 2.0 * o_ * o_ * v_ * v_ * v_ * v_);
+        gemm_f32_fd(c.s, o * o, v * v, v * v, 1.0, tau32.f(), o * o, bY.f(), v * v, 1.0, P.p, o * o, P.p, o * o, 1);
+      }
     }
   }
   contract(c, 1.0, Lvv, "ac", t2, "ijcb", 1.0, P, "ijab");
   contract(c, -1.0, Loo, "ki", t2, "kjab", 1.0, P, "ijab");
   {
-    Timer tr(c, c.lowp ? "ring_fp32" : "ring", 6.0 * 2.0 * o_ * o_ * o_ * v_ * v_ * v_);
+    Timer tr(c, c.lowp == 2 ? "ring_bf16x3" : (c.lowp ? "ring_fp32" : "ring"), 6.0 * 2.0 * o_ * o_ * o_ * v_ * v_ * v_);
     Buf bWa = dalloc(c, c.n2()), bWb = dalloc(c, c.n2()), bXX = dalloc(c, c.n2());
     TView Wvoov = tv(bWa.d(), {v, o, o, v}), Wvovo = tv(bWb.d(), {v, o, v, o});
     // Wvoov[a,k,i,c]
@@ -542,6 +561,28 @@ static void mp2_guess(Ctx& c) {
   fill_f64(c.s, c.n2(), 0.0, zero.d());
   jacobi_update(c.s, c.o, c.v, true, c.eo.d(), c.ev.d(), c.T2.d(), zero.d(), nullptr, false, c.partials.d(), &nb);
   c.have_amp = true;
+}
+
+// learned guess: MP2 amplitudes plus the network's symmetrised correction, evaluated in chunks
+static void learned_guess(Ctx& c) {
+  FH_CHECK(c.have_model, -6, "no T2-guess model: call fh_set_t2_model after fh_set_df");
+  mp2_guess(c);
+  Timer t(c, "t2_guess_learned", 0.0);
+  const int h = c.mlp_h;
+  const int64_t n = c.n2();
+  const int64_t C = std::min<int64_t>(n, (int64_t)1 << 20);
+  Buf X(c.pool.get(), (size_t)(6 * C) * sizeof(float)), A(c.pool.get(), (size_t)(3 * h * C) * sizeof(float));
+  Buf Y(c.pool.get(), (size_t)n * sizeof(float));
+  float* A1 = A.f();
+  float* A2 = A1 + (int64_t)h * C;
+  float* A3 = A2 + (int64_t)h * C;
+  for (int64_t e0 = 0; e0 < n; e0 += C) {
+    const int64_t m = std::min<int64_t>(C, n - e0);
+    t2_features(c.s, (int)c.o, (int)c.v, e0, m, c.T2.d(), c.eo.d(), c.ev.d(), c.pair_d.d(), c.mlp_xm.f(),
+                c.mlp_xs.f(), X.f());
+    mlp_forward(c.s, h, c.mlp_P.f(), m, X.f(), A1, A2, A3, Y.f() + e0);
+  }
+  t2_learned_apply(c.s, (int)c.o, (int)c.v, c.mlp_ys, Y.f(), c.T2.d());
 }
 
 // ------------------------------------------------------------------ DIIS (DIIS.jl:21-96)
@@ -621,7 +662,8 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   FH_CHECK(c.have_amp, -6, "no starting amplitudes");
   FH_CHECK(op.cc_max_iter >= 0 && op.cc_ndiis >= 1 && op.cc_ndiis <= 8 && op.cc_diis_relax >= 1, -1,
            "invalid CC options");
-  FH_CHECK(op.precision_mode == 0 || op.precision_mode == 1, -1, "precision_mode must be 0 (FP64) or 1 (FP32 mixed)");
+  FH_CHECK(op.precision_mode >= 0 && op.precision_mode <= 2, -1,
+           "precision_mode must be 0 (FP64), 1 (FP32 then FP64) or 2 (BF16x3, FP32, then FP64)");
   const int64_t n1 = c.nov(), n2 = c.n2();
   const double epsT = 2.220446049250313e-16;
   double e_conv = op.cc_e_conv, max_rms = op.cc_max_rms;
@@ -631,11 +673,13 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   const double dp = op.cc_damp_ratio;
   std::unique_ptr<DIIS> dm1(new DIIS(c, n1, op.cc_ndiis, f32)), dm2(new DIIS(c, n2, op.cc_ndiis, f32));
   c.iters.clear();
-  // mixed precision: phase 1 runs the residual's contractions in FP32 until the switch
-  // thresholds are met, then the FP64 phase polishes (fresh DIIS subspace)
-  bool mixed = op.precision_mode == 1;
-  int polish_done = 0, niter_mixed = 0;
+  // mixed precision: the residual's contractions run at `level` (2 split-BF16x3, 1 FP32,
+  // 0 FP64).  A reduced-precision phase ends when |dE| and rms fall below its switch thresholds;
+  // each switch starts a fresh DIIS subspace, and the FP64 phase polishes to the final criteria.
+  int level = op.precision_mode;
+  int polish_done = 0, niter_mixed = 0, niter_bf3 = 0;
   const double sw_e = std::max(op.mixed_e_switch, e_conv), sw_rms = std::max(op.mixed_rms_switch, max_rms);
+  const double sw_e2 = std::max(op.bf3_e_switch, sw_e), sw_rms2 = std::max(op.bf3_rms_switch, sw_rms);
 
   Buf nT1 = std::move(c.T1), nT2 = std::move(c.T2);     // newT (guess)
   Buf oT1 = dalloc(c, n1), oT2 = dalloc(c, n2);         // T
@@ -645,19 +689,19 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   int ite = 1;
   if (op.verbose) std::printf("%10s    %15s    %12s    %12s    %10s    %5s\n", "Iteration", "CC Energy", "dE", "Max RMS", "time (s)", "diis");
   while (true) {
-    if (mixed && std::fabs(dE) < sw_e && rms < sw_rms) {
-      mixed = false;                                       // switch to FP64 polish
+    while (level > 0 && std::fabs(dE) < (level == 2 ? sw_e2 : sw_e) && rms < (level == 2 ? sw_rms2 : sw_rms)) {
+      --level;                                             // next precision level
       dm1.reset(new DIIS(c, n1, op.cc_ndiis, f32));
       dm2.reset(new DIIS(c, n2, op.cc_ndiis, f32));
       dE = 1.0;
       rms = 1.0;
     }
-    if (!mixed) {
+    if (level == 0) {
       if (!(std::fabs(dE) > e_conv || rms > max_rms)) break;                                   // :178
-      if (op.precision_mode == 1 && op.polish_iters >= 0 && polish_done >= op.polish_iters) break;
+      if (op.precision_mode > 0 && op.polish_iters >= 0 && polish_done >= op.polish_iters) break;
     }
     if (ite > op.cc_max_iter) break;
-    c.lowp = mixed;
+    c.lowp = level;
     auto t0 = std::chrono::steady_clock::now();
     std::swap(oT1, nT1);                                                                        // :185-186
     std::swap(oT2, nT2);
@@ -671,7 +715,7 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
     }
     double r1 = 0.0, r2 = 0.0;
     update_amp(c, oT1.d(), oT2.d(), nT1.d(), nT2.d(), e1.p, e2.p, f32, &r1, &r2);               // :190-194
-    c.lowp = false;
+    c.lowp = 0;
     bool diis = false;
     if (do_diis) {
       dm1->push(nT1.d(), std::move(e1));                                                        // :199-202
@@ -690,21 +734,23 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
     Ecc = energy(c, nT1.d(), nT2.d());                                                          // :218
     rms = std::max(r1, r2);
     dE = Ecc - oldE;
-    if (mixed) ++niter_mixed;
-    else if (op.precision_mode == 1) ++polish_done;
+    if (level > 0) ++niter_mixed;
+    if (level == 2) ++niter_bf3;
+    if (level == 0 && op.precision_mode > 0) ++polish_done;
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c.iters.push_back({ite, Ecc, dE, rms, ms, diis});
-    if (op.verbose) std::printf("    %-5d    %-15.10f    %+12.10f    %-12.10f    %-10.5f    %s%s\n", ite, Ecc, dE, rms, ms / 1e3, diis ? "true" : "false", mixed ? " fp32" : "");
+    if (op.verbose) std::printf("    %-5d    %-15.10f    %+12.10f    %-12.10f    %-10.5f    %s%s\n", ite, Ecc, dE, rms, ms / 1e3, diis ? "true" : "false", level == 2 ? " bf16x3" : (level == 1 ? " fp32" : ""));
     FH_CHECK(std::isfinite(Ecc) && std::isfinite(rms), -5, "non-finite CC energy or amplitudes at iteration " + std::to_string(ite));
     ++ite;
   }
-  c.lowp = false;
+  c.lowp = 0;
   res.correlation = Ecc;
   res.e_conv = dE;
   res.t_conv = rms;
   res.niter = ite - 1;
   res.niter_mixed = niter_mixed;
-  res.converged = (!mixed && std::fabs(dE) < e_conv && rms < max_rms) ? 1 : 0;                 // :231
+  res.niter_bf3 = niter_bf3;
+  res.converged = (level == 0 && std::fabs(dE) < e_conv && rms < max_rms) ? 1 : 0;            // :231
   c.T1 = std::move(nT1);
   c.T2 = std::move(nT2);
   c.have_amp = true;
@@ -768,27 +814,33 @@ __global__ void k_accumulate(int64_t n, const double* __restrict__ partials, dou
 // one W-build GEMM of the (T) driver (tag 2), FP64 or FP32 data
 static void tgemm(Ctx& c, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, int64_t sA, const double* B,
                   int64_t ldb, int64_t sB, double beta, double* W, int64_t ldd, int64_t sD, int64_t batch,
-                  const OutMap* map) {
+                  const OutMap* map, bool) {
   gemm_f64(c.s, false, false, M, N, K, 1.0, A, lda, sA, B, ldb, sB, beta, beta != 0.0 ? W : nullptr, ldd, sD, W, ldd, sD,
            batch, 2, nullptr, 0, map, false);
 }
 static void tgemm(Ctx& c, int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, int64_t sA, const float* B,
                   int64_t ldb, int64_t sB, double beta, float* W, int64_t ldd, int64_t sD, int64_t batch,
-                  const OutMap* map) {
-  gemm_f32_ff(c.s, M, N, K, 1.0, A, lda, sA, B, ldb, sB, beta, beta != 0.0 ? W : nullptr, ldd, sD, W, ldd, sD, batch, 2,
-              map);
+                  const OutMap* map, bool bf3) {
+  if (bf3)
+    gemm_bf3<float, float>(c.s, false, false, M, N, K, 1.0, A, lda, sA, B, ldb, sB, beta, beta != 0.0 ? W : nullptr, ldd,
+                           sD, W, ldd, sD, batch, 2, map);
+  else
+    gemm_f32_ff(c.s, M, N, K, 1.0, A, lda, sA, B, ldb, sB, beta, beta != 0.0 ? W : nullptr, ldd, sD, W, ldd, sD, batch,
+                2, map);
 }
 
-// (T) with W-build operands and W stored as T (double: FP64 path; float: FP32 MFMA path)
+// (T) with W-build operands and W stored as T (double: FP64 path; float: FP32 MFMA path, or
+// split-BF16x3 MFMA when bf3)
 template <typename T>
-static double triples_t(Ctx& c) {
+static double triples_t(Ctx& c, bool bf3) {
   prepare(c);
   FH_CHECK(c.have_amp, -6, "no amplitudes for (T)");
   const bool lowp = std::is_same<T, float>::value;
   const int64_t o = c.o, v = c.v, vv = v * v, nd = v + o;
   const double o_ = (double)o, v_ = (double)v;
   const double ntrip = o_ * (o_ + 1) * (o_ + 2) / 6.0;
-  Timer tall(c, lowp ? "triples_fp32" : "triples", ntrip * 12.0 * (v_ * v_ * v_ * v_ + o_ * v_ * v_ * v_));
+  const std::string sfx = bf3 ? "_bf16x3" : (lowp ? "_fp32" : "");
+  Timer tall(c, "triples" + sfx, ntrip * 12.0 * (v_ * v_ * v_ * v_ + o_ * v_ * v_ * v_));
   // AV[(x,y),n,d'] and the row-swapped copy AVs[(y,x),n,d'] share one allocation;
   // BVq (Bcat_{p,q} stored [d',z,q,p]) and BVp (stored [d',z,p,q]) likewise
   const int64_t avsz = vv * o * nd, bvsz = nd * v * o * o;
@@ -834,24 +886,24 @@ static double triples_t(Ctx& c) {
       const int64_t kend = (i == j) ? j : j + 1;  // i=j=k has zero weight (ijk.jl:133)
       for (int64_t k0 = 0; k0 < kend; k0 += KC) {
         const int64_t nk = std::min<int64_t>(KC, kend - k0);
-        std::unique_ptr<Timer> tg(new Timer(c, lowp ? "triples_gemm_fp32" : "triples_gemm",
+        std::unique_ptr<Timer> tg(new Timer(c, "triples_gemm" + sfx,
                                             12.0 * (double)vv * (double)v * (double)nd * (double)nk));
         // The six permutation classes of ijk2.jl:109-153, G = AV_n (rows (x,y)) * Bcat_{p,q}
         // (cols z), accumulated into W[a,b,c,kk] (W[a,b,c] += G[class-permuted (a,b,c)]).
         // abc: n=j, B_{i,k}, x=a y=b z=c
-        tgemm(c, vv, v * nk, nd, AV + vv * j, vv * o, 0, BVq + bstride * (k0 + o * i), nd, 0, 0.0, W, vv, 0, 1, nullptr);
+        tgemm(c, vv, v * nk, nd, AV + vv * j, vv * o, 0, BVq + bstride * (k0 + o * i), nd, 0, 0.0, W, vv, 0, 1, nullptr, bf3);
         // bac: n=i, B_{j,k}, x=b y=a z=c  (row-swapped AV: rows (a,b))
-        tgemm(c, vv, v * nk, nd, AVs + vv * i, vv * o, 0, BVq + bstride * (k0 + o * j), nd, 0, 1.0, W, vv, 0, 1, nullptr);
+        tgemm(c, vv, v * nk, nd, AVs + vv * i, vv * o, 0, BVq + bstride * (k0 + o * j), nd, 0, 1.0, W, vv, 0, 1, nullptr, bf3);
         // cab: n=i, B_{k,j}, x=c y=a z=b  (rows (a,c), cols (b,kk))
-        tgemm(c, vv, v * nk, nd, AVs + vv * i, vv * o, 0, BVp + bstride * (k0 + o * j), nd, 0, 1.0, W, vv, 0, 1, &m_cab);
+        tgemm(c, vv, v * nk, nd, AVs + vv * i, vv * o, 0, BVp + bstride * (k0 + o * j), nd, 0, 1.0, W, vv, 0, 1, &m_cab, bf3);
         // cba: n=j, B_{k,i}, x=c y=b z=a  (rows (b,c), cols (a,kk))
-        tgemm(c, vv, v * nk, nd, AVs + vv * j, vv * o, 0, BVp + bstride * (k0 + o * i), nd, 0, 1.0, W, vv, 0, 1, &m_cba);
+        tgemm(c, vv, v * nk, nd, AVs + vv * j, vv * o, 0, BVp + bstride * (k0 + o * i), nd, 0, 1.0, W, vv, 0, 1, &m_cba, bf3);
         // acb: n=k, B_{i,j}, x=a y=c z=b  (batched over k: rows (a,c), col b)
-        tgemm(c, vv, v, nd, AV + vv * k0, vv * o, vv, BVp + bstride * (i + o * j), nd, 0, 1.0, W, vv, v3, nk, &m_acb);
+        tgemm(c, vv, v, nd, AV + vv * k0, vv * o, vv, BVp + bstride * (i + o * j), nd, 0, 1.0, W, vv, v3, nk, &m_acb, bf3);
         // bca: n=k, B_{j,i}, x=b y=c z=a  (batched over k: rows (b,c), col a)
-        tgemm(c, vv, v, nd, AV + vv * k0, vv * o, vv, BVp + bstride * (j + o * i), nd, 0, 1.0, W, vv, v3, nk, &m_bca);
+        tgemm(c, vv, v, nd, AV + vv * k0, vv * o, vv, BVp + bstride * (j + o * i), nd, 0, 1.0, W, vv, v3, nk, &m_bca, bf3);
         tg.reset();
-        Timer te(c, lowp ? "triples_energy_fp32" : "triples_energy", 0.0);
+        Timer te(c, "triples_energy" + sfx, 0.0);
         int64_t nb = 0;
         triples_energy(c.s, (int)o, (int)v, (int)i, (int)j, (int)k0, (int)nk, W, static_cast<const int*>(orbits.p),
                        T1p.d(), vvoo.d(), c.eo.d(), c.ev.d(), part.d(), &nb);
@@ -864,7 +916,10 @@ static double triples_t(Ctx& c) {
   return Et;
 }
 
-static double triples(Ctx& c, bool lowp) { return lowp ? triples_t<float>(c) : triples_t<double>(c); }
+// prec_mode 0: FP64; 1: FP32 MFMA W build; 2: split-BF16x3 MFMA W build
+static double triples(Ctx& c, int prec_mode) {
+  return prec_mode == 0 ? triples_t<double>(c, false) : triples_t<float>(c, prec_mode == 2);
+}
 
 }  // namespace fh
 
@@ -914,6 +969,8 @@ void fh_default_opts(fh_cc_opts* p) {
   p->polish_iters = -1;
   p->mixed_e_switch = 1e-8;
   p->mixed_rms_switch = 1e-7;
+  p->bf3_e_switch = 1e-6;
+  p->bf3_rms_switch = 1e-5;
 }
 
 const char* fh_last_error(void) { return g_last_error.c_str(); }
@@ -960,6 +1017,7 @@ int fh_set_df(fh_ctx* x, int o, int v, int naux, const double* eps_o, const doub
     c.have_amp = false;
     c.OOOO.reset(); c.OOOV.reset(); c.OOVV.reset(); c.OVOV.reset(); c.OVVV.reset(); c.Lovov.reset();
     c.Vperm.reset(); c.Vperm32.reset(); c.Pt.reset(); c.T1.reset(); c.T2.reset();
+    c.have_model = false; c.pair_d.reset();
     c.o = o; c.v = v; c.naux = naux;
     const int64_t Q = naux;
     c.eo = dalloc(c, o); c.ev = dalloc(c, v);
@@ -995,11 +1053,12 @@ static void upload_amp(Ctx& c, const double* T1, const double* T2) {
 int fh_t2_guess(fh_ctx* x, int kind, double* T1, double* T2) {
   return guarded([&] {
     FH_CHECK(x != nullptr, -1, "null context");
-    FH_CHECK(kind == 0, -1, "fh_t2_guess: only kind 0 (MP2) is available");
+    FH_CHECK(kind == 0 || kind == 1, -1, "fh_t2_guess: kind must be 0 (MP2) or 1 (learned)");
     DeviceGuard g(x->c.device);
     Ctx& c = x->c;
     require_df(c);
-    mp2_guess(c);
+    if (kind == 1) learned_guess(c);
+    else mp2_guess(c);
     if (T1) d2h(c, T1, c.T1.d(), c.nov());
     if (T2) d2h(c, T2, c.T2.d(), c.n2());
     FH_HIP(hipStreamSynchronize(c.s));
@@ -1071,13 +1130,14 @@ int fh_ccsd_solve(fh_ctx* x, const fh_cc_opts* opts, double* T1, double* T2, fh_
 int fh_triples(fh_ctx* x, int prec_mode, const double* T1, const double* T2, double* Et) {
   return guarded([&] {
     FH_CHECK(x && Et, -1, "null argument");
-    FH_CHECK(prec_mode == 0 || prec_mode == 1, -1, "fh_triples: prec_mode must be 0 (FP64) or 1 (FP32 W build)");
+    FH_CHECK(prec_mode >= 0 && prec_mode <= 2, -1,
+             "fh_triples: prec_mode must be 0 (FP64), 1 (FP32 W build) or 2 (split-BF16x3 W build)");
     FH_CHECK((T1 == nullptr) == (T2 == nullptr), -1, "T1 and T2 must both be given or both NULL");
     DeviceGuard g(x->c.device);
     Ctx& c = x->c;
     require_df(c);
     if (T1) upload_amp(c, T1, T2);
-    *Et = triples(c, prec_mode == 1);
+    *Et = triples(c, prec_mode);
     flush_timers(c);
     FH_CHECK(std::isfinite(*Et), -5, "non-finite (T) energy");
   });
@@ -1111,6 +1171,220 @@ int fh_stats(fh_ctx* x, char* json, size_t cap) {
     const std::string out = s.str();
     FH_CHECK(out.size() + 1 <= cap, -1, "fh_stats: buffer too small (" + std::to_string(out.size() + 1) + " bytes needed)");
     std::memcpy(json, out.c_str(), out.size() + 1);
+  });
+}
+
+// ---------------------------------------------------------------- learned T2 guess network
+struct fh_mlp {
+  int device = 0;
+  hipStream_t s = nullptr;
+  std::unique_ptr<Pool> pool;  // declared before the Bufs: destroyed after them
+  int h = 0;
+  int64_t step = 0;
+  Buf P, G, m, v;
+};
+
+static void mlp_upload(fh_mlp* n, void* dst, const void* src, size_t bytes) {
+  FH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, n->s));
+}
+
+int fh_mlp_create(fh_mlp** out, int device, int hidden, unsigned long long seed) {
+  return guarded([&] {
+    FH_CHECK(out != nullptr, -1, "null argument");
+    FH_CHECK(hidden > 0 && 256 % hidden == 0, -1, "fh_mlp_create: hidden width must divide 256");
+    *out = nullptr;
+    DeviceGuard g(device);
+    std::unique_ptr<fh_mlp> n(new fh_mlp());
+    n->device = device;
+    FH_HIP(hipSetDevice(device));
+    FH_HIP(hipStreamCreate(&n->s));
+    n->pool.reset(new Pool(device));
+    n->h = hidden;
+    const size_t np = mlp_nparams(hidden);
+    n->P = Buf(n->pool.get(), np * sizeof(float));
+    n->G = Buf(n->pool.get(), np * sizeof(float));
+    n->m = Buf(n->pool.get(), np * sizeof(float));
+    n->v = Buf(n->pool.get(), np * sizeof(float));
+    // Glorot-uniform weights, zero biases
+    std::vector<float> p(np, 0.0f);
+    std::mt19937_64 rng(seed);
+    const MlpOffsets o = mlp_offsets(hidden);
+    auto glorot = [&](size_t off, int fan_out, int fan_in) {
+      std::uniform_real_distribution<float> u(-1.0f, 1.0f);
+      const float a = std::sqrt(6.0f / (float)(fan_in + fan_out));
+      for (size_t k = 0; k < (size_t)fan_out * fan_in; ++k) p[off + k] = a * u(rng);
+    };
+    glorot(o.W1, hidden, 6);
+    glorot(o.W2, hidden, hidden);
+    glorot(o.W3, hidden, hidden);
+    glorot(o.w4, 1, hidden);
+    mlp_upload(n.get(), n->P.p, p.data(), np * sizeof(float));
+    std::vector<float> z(np, 0.0f);
+    mlp_upload(n.get(), n->m.p, z.data(), np * sizeof(float));
+    mlp_upload(n.get(), n->v.p, z.data(), np * sizeof(float));
+    FH_HIP(hipStreamSynchronize(n->s));
+    *out = n.release();
+  });
+}
+
+size_t fh_mlp_num_params(const fh_mlp* n) { return n ? mlp_nparams(n->h) : 0; }
+
+int fh_mlp_get_params(fh_mlp* n, float* params) {
+  return guarded([&] {
+    FH_CHECK(n != nullptr && params != nullptr, -1, "null argument");
+    DeviceGuard g(n->device);
+    FH_HIP(hipMemcpyAsync(params, n->P.p, mlp_nparams(n->h) * sizeof(float), hipMemcpyDeviceToHost, n->s));
+    FH_HIP(hipStreamSynchronize(n->s));
+  });
+}
+
+int fh_mlp_set_params(fh_mlp* n, const float* params) {
+  return guarded([&] {
+    FH_CHECK(n != nullptr && params != nullptr, -1, "null argument");
+    DeviceGuard g(n->device);
+    for (size_t k = 0; k < mlp_nparams(n->h); ++k)
+      FH_CHECK(std::isfinite(params[k]), -5, "fh_mlp_set_params: non-finite parameter");
+    mlp_upload(n, n->P.p, params, mlp_nparams(n->h) * sizeof(float));
+    n->step = 0;
+    std::vector<float> z(mlp_nparams(n->h), 0.0f);
+    mlp_upload(n, n->m.p, z.data(), z.size() * sizeof(float));
+    mlp_upload(n, n->v.p, z.data(), z.size() * sizeof(float));
+    FH_HIP(hipStreamSynchronize(n->s));
+  });
+}
+
+int fh_mlp_forward(fh_mlp* n, long long ns, const float* X, float* Y) {
+  return guarded([&] {
+    FH_CHECK(n != nullptr && X != nullptr && Y != nullptr && ns >= 0, -1, "invalid argument");
+    DeviceGuard g(n->device);
+    if (ns == 0) return;
+    const int h = n->h;
+    const int64_t C = std::min<int64_t>(ns, (int64_t)1 << 20);
+    Buf dX(n->pool.get(), (size_t)(6 * C) * sizeof(float)), A(n->pool.get(), (size_t)(3 * h * C) * sizeof(float));
+    Buf dY(n->pool.get(), (size_t)C * sizeof(float));
+    for (int64_t e0 = 0; e0 < ns; e0 += C) {
+      const int64_t m = std::min<int64_t>(C, ns - e0);
+      mlp_upload(n, dX.p, X + 6 * e0, (size_t)(6 * m) * sizeof(float));
+      mlp_forward(n->s, h, n->P.f(), m, dX.f(), A.f(), A.f() + h * C, A.f() + 2 * h * C, dY.f());
+      FH_HIP(hipMemcpyAsync(Y + e0, dY.p, (size_t)m * sizeof(float), hipMemcpyDeviceToHost, n->s));
+    }
+    FH_HIP(hipStreamSynchronize(n->s));
+  });
+}
+
+// one batch on the device: forward + backward into n->G; batch loss sum (B*L) into *lsum (device)
+static void mlp_batch(fh_mlp* n, int64_t B, const float* X, const float* T, Buf& A, Buf& dy, Buf& dAZ, Buf& part,
+                      Buf& lpart, double* lsum) {
+  const int h = n->h;
+  float* A1 = A.f();
+  float* A2 = A1 + (int64_t)h * B;
+  float* A3 = A2 + (int64_t)h * B;
+  float* y = A3 + (int64_t)h * B;
+  mlp_forward(n->s, h, n->P.f(), B, X, A1, A2, A3, y);
+  int64_t nl = 0;
+  mlp_backward(n->s, h, n->P.f(), B, X, T, A1, A2, A3, y, dy.f(), dAZ.f(), dAZ.f() + (int64_t)h * B, n->G.f(), part.f(),
+               lpart.d(), &nl);
+  sum_partials(n->s, nl, lpart.d(), lsum);
+}
+
+struct MlpWork {
+  Buf A, dy, dAZ, part, lpart;
+  MlpWork(fh_mlp* n, int64_t B)
+      : A(n->pool.get(), (size_t)((3 * n->h + 1) * B) * sizeof(float)),
+        dy(n->pool.get(), (size_t)B * sizeof(float)),
+        dAZ(n->pool.get(), (size_t)(2 * n->h * B) * sizeof(float)),
+        part(n->pool.get(), (size_t)mlp_partial_elems(n->h, B) * sizeof(float)),
+        lpart(n->pool.get(), (size_t)65536 * sizeof(double)) {}
+};
+
+int fh_mlp_loss_grad(fh_mlp* n, long long ns, const float* X, const float* T, double* loss, float* grad) {
+  return guarded([&] {
+    FH_CHECK(n != nullptr && X != nullptr && T != nullptr && ns > 0, -1, "invalid argument");
+    DeviceGuard g(n->device);
+    Buf dX(n->pool.get(), (size_t)(6 * ns) * sizeof(float)), dT(n->pool.get(), (size_t)ns * sizeof(float));
+    Buf ls(n->pool.get(), sizeof(double));
+    mlp_upload(n, dX.p, X, (size_t)(6 * ns) * sizeof(float));
+    mlp_upload(n, dT.p, T, (size_t)ns * sizeof(float));
+    MlpWork w(n, ns);
+    mlp_batch(n, ns, dX.f(), dT.f(), w.A, w.dy, w.dAZ, w.part, w.lpart, ls.d());
+    double l = 0.0;
+    FH_HIP(hipMemcpyAsync(&l, ls.p, sizeof(double), hipMemcpyDeviceToHost, n->s));
+    if (grad) FH_HIP(hipMemcpyAsync(grad, n->G.p, mlp_nparams(n->h) * sizeof(float), hipMemcpyDeviceToHost, n->s));
+    FH_HIP(hipStreamSynchronize(n->s));
+    if (loss) *loss = l / (double)ns;
+  });
+}
+
+int fh_mlp_train(fh_mlp* n, long long ns, const float* X, const float* T, int epochs, int batch, float lr,
+                 unsigned long long seed, double* epoch_loss) {
+  return guarded([&] {
+    FH_CHECK(n != nullptr && X != nullptr && T != nullptr && ns > 0 && epochs >= 0 && batch > 0 && lr > 0.0f, -1,
+             "invalid argument");
+    DeviceGuard g(n->device);
+    Buf dX(n->pool.get(), (size_t)(6 * ns) * sizeof(float)), dT(n->pool.get(), (size_t)ns * sizeof(float));
+    mlp_upload(n, dX.p, X, (size_t)(6 * ns) * sizeof(float));
+    mlp_upload(n, dT.p, T, (size_t)ns * sizeof(float));
+    const int64_t B = std::min<int64_t>(batch, ns);
+    const int64_t nbat = (ns + B - 1) / B;
+    MlpWork w(n, B);
+    Buf Xb(n->pool.get(), (size_t)(6 * B) * sizeof(float)), Tb(n->pool.get(), (size_t)B * sizeof(float));
+    Buf perm(n->pool.get(), (size_t)ns * sizeof(int64_t)), ls(n->pool.get(), (size_t)nbat * sizeof(double));
+    std::vector<int64_t> hp(ns);
+    std::vector<double> hl(nbat);
+    std::mt19937_64 rng(seed);
+    const size_t np = mlp_nparams(n->h);
+    for (int ep = 0; ep < epochs; ++ep) {
+      for (int64_t k = 0; k < ns; ++k) hp[k] = k;
+      std::shuffle(hp.begin(), hp.end(), rng);
+      mlp_upload(n, perm.p, hp.data(), (size_t)ns * sizeof(int64_t));
+      for (int64_t b = 0; b < nbat; ++b) {
+        const int64_t off = b * B, m = std::min<int64_t>(B, ns - off);
+        mlp_gather(n->s, m, off, static_cast<const int64_t*>(perm.p), dX.f(), dT.f(), Xb.f(), Tb.f());
+        mlp_batch(n, m, Xb.f(), Tb.f(), w.A, w.dy, w.dAZ, w.part, w.lpart, ls.d() + b);
+        mlp_adam(n->s, (int64_t)np, n->P.f(), n->G.f(), n->m.f(), n->v.f(), lr, ++n->step);
+      }
+      FH_HIP(hipMemcpyAsync(hl.data(), ls.p, (size_t)nbat * sizeof(double), hipMemcpyDeviceToHost, n->s));
+      FH_HIP(hipStreamSynchronize(n->s));
+      double tot = 0.0;
+      for (double q : hl) tot += q;
+      FH_CHECK(std::isfinite(tot), -5, "fh_mlp_train: non-finite loss");
+      if (epoch_loss) epoch_loss[ep] = tot / (double)ns;
+    }
+  });
+}
+
+void fh_mlp_destroy(fh_mlp* n) {
+  if (!n) return;
+  DeviceGuard g(n->device);
+  (void)hipStreamSynchronize(n->s);
+  hipStream_t s = n->s;
+  delete n;
+  (void)hipStreamDestroy(s);
+}
+
+int fh_set_t2_model(fh_ctx* x, const fh_mlp* net, const float* x_mean, const float* x_scale, double y_scale,
+                    const double* pair_dist) {
+  return guarded([&] {
+    FH_CHECK(x != nullptr && net != nullptr && x_mean != nullptr && x_scale != nullptr, -1, "null argument");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    require_df(c);
+    FH_CHECK(std::isfinite(y_scale), -1, "fh_set_t2_model: non-finite y_scale");
+    const size_t np = mlp_nparams(net->h);
+    c.mlp_P = Buf(c.pool.get(), np * sizeof(float));
+    FH_HIP(hipStreamSynchronize(net->s));
+    FH_HIP(hipMemcpyAsync(c.mlp_P.p, net->P.p, np * sizeof(float), hipMemcpyDeviceToDevice, c.s));
+    c.mlp_xm = Buf(c.pool.get(), 6 * sizeof(float));
+    c.mlp_xs = Buf(c.pool.get(), 6 * sizeof(float));
+    FH_HIP(hipMemcpyAsync(c.mlp_xm.p, x_mean, 6 * sizeof(float), hipMemcpyHostToDevice, c.s));
+    FH_HIP(hipMemcpyAsync(c.mlp_xs.p, x_scale, 6 * sizeof(float), hipMemcpyHostToDevice, c.s));
+    c.pair_d = dalloc(c, c.o * c.o);
+    if (pair_dist) FH_HIP(hipMemcpyAsync(c.pair_d.p, pair_dist, (size_t)(c.o * c.o) * sizeof(double), hipMemcpyHostToDevice, c.s));
+    else fill_f64(c.s, c.o * c.o, 0.0, c.pair_d.d());
+    FH_HIP(hipStreamSynchronize(c.s));
+    c.mlp_h = net->h;
+    c.mlp_ys = y_scale;
+    c.have_model = true;
   });
 }
 

@@ -327,4 +327,20 @@ void gemm_f32_ff(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, c
   else gemm_nn_f32<0>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, map);
 }
 
+// FP32 operands and output, any transposition (the T2-guess network's layers; tag 3)
+void gemm_f32(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const float* A,
+              int64_t lda, const float* B, int64_t ldb, double beta, float* C, int64_t ldc) {
+  if (M <= 0 || N <= 0) return;
+  check_args(M, K, N, ta, tb, lda, ldb, ldc, beta, C, C, ldc, nullptr);
+  const OutMap om{};
+#define FH_F32(TA_, TB_) \
+  pick<TA_, TB_, 3, true, float, float>(s, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, C, ldc, 0, 1, 0, om)
+  if (ta) {
+    if (tb) FH_F32(true, true); else FH_F32(true, false);
+  } else {
+    if (tb) FH_F32(false, true); else FH_F32(false, false);
+  }
+#undef FH_F32
+}
+
 }  // namespace fh

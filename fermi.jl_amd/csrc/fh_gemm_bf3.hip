@@ -1,0 +1,333 @@
+// Split-BF16x3 GEMM on CDNA4 matrix cores: D = alpha * op(A) * op(B) + beta * C, column-major.
+//
+// Every FP32 (or FP64) operand x is split when it is staged into LDS into two bfloat16 parts,
+// hi = bf16(x) and lo = bf16(x - hi) (both round-to-nearest-even; x - hi is exact in FP32), and
+// the product is formed from three bf16 MFMA products with FP32 accumulation:
+//     A*B ~= Ahi*Bhi + Ahi*Blo + Alo*Bhi
+// (the dropped Alo*Blo term and the rounding of lo leave a relative error of about 2^-16 per
+// product; Henry, Tang and Heinecke, "Leveraging the bfloat16 artificial intelligence datatype
+// for higher-precision computations", ARITH 2019).  Three v_mfma_f32_32x32x16_bf16 issue in the
+// time of 3/16 of the FP32 MFMA work of the same tile.
+//
+// Workgroup: 256 threads = 4 wavefronts (2x2), 128x128 output tile, K staged 32 at a time
+// through double-buffered LDS (hi and lo planes of both operands, 80 KB: two workgroups per CU).
+// Each wavefront owns 64x64 = 2x2 blocks of 32x32 accumulators.  The MFMA is issued with the B
+// tile as its first operand, so the accumulator's lane index runs along the rows of D and the
+// epilogue stores are coalesced along the column-major output.
+//
+// Global loads: four consecutive elements per thread (dwordx4 for FP32) when the leading
+// dimensions, batch strides and base addresses allow it (VEC), scalar loads otherwise.
+// Tile order: the XCD-aware remap and the L2-grouped raster of the FP64/FP32 kernel.
+#include "fh_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <type_traits>
+
+namespace fh {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+namespace {
+
+constexpr int kBM = 128, kBN = 128, kBK = 32;
+constexpr int kRow = kBK + 8;  // LDS row of one operand tile: 32 bf16 + 16 bytes of padding
+
+// 4 values of an operand, as FP32
+struct V4 {
+  float x[4];
+};
+
+// two FP32 values -> packed (hi, lo) bf16 pairs
+__device__ inline void split2(float a, float b, uint32_t& hi, uint32_t& lo) {
+  const f32x2 v = {a, b};
+  const bf16x2 h = __builtin_convertvector(v, bf16x2);
+  uint32_t hb;
+  __builtin_memcpy(&hb, &h, 4);
+  const float ha = __uint_as_float(hb << 16), hbf = __uint_as_float(hb & 0xffff0000u);
+  const f32x2 r = {a - ha, b - hbf};
+  const bf16x2 l = __builtin_convertvector(r, bf16x2);
+  uint32_t lb;
+  __builtin_memcpy(&lb, &l, 4);
+  hi = hb;
+  lo = lb;
+}
+
+// split 4 values and write them to LDS rows (8 bytes per plane)
+__device__ inline void put4(uint16_t* hi_plane, uint16_t* lo_plane, int off, const V4& v) {
+  uint32_t h0, l0, h1, l1;
+  split2(v.x[0], v.x[1], h0, l0);
+  split2(v.x[2], v.x[3], h1, l1);
+  *reinterpret_cast<uint2*>(hi_plane + off) = make_uint2(h0, h1);
+  *reinterpret_cast<uint2*>(lo_plane + off) = make_uint2(l0, l1);
+}
+
+__device__ inline int clamp4(int64_t n) { return n <= 0 ? 0 : (n >= 4 ? 4 : (int)n); }
+
+template <typename TI, bool VEC>
+__device__ inline V4 load4(const TI* p, bool full, int n_ok) {
+  V4 v;
+  if constexpr (VEC) {
+    if (full) {
+      if constexpr (std::is_same<TI, float>::value) {
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        v.x[0] = q.x; v.x[1] = q.y; v.x[2] = q.z; v.x[3] = q.w;
+      } else {
+        const double2 q0 = reinterpret_cast<const double2*>(p)[0], q1 = reinterpret_cast<const double2*>(p)[1];
+        v.x[0] = (float)q0.x; v.x[1] = (float)q0.y; v.x[2] = (float)q1.x; v.x[3] = (float)q1.y;
+      }
+      return v;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v.x[i] = i < n_ok ? (float)p[i] : 0.0f;
+  return v;
+}
+
+}  // namespace
+
+// Operand tile layout in LDS: plane[row][k] (k fastest, kRow bf16 per row), row = m for A, n for B.
+// "row-contiguous" operands (A with TA = false, B with TB = true) are read 4 rows x 4 k per thread
+// and transposed in registers; "k-contiguous" operands are read 4 k of one row per load.
+template <bool TA, bool TB, int TAG, typename TI, typename TO, bool VEC>
+__global__ __launch_bounds__(256, 2) void k_gemm_bf3(int64_t M, int64_t N, int64_t K, double alpha,
+                                                     const TI* __restrict__ A, int64_t lda, int64_t sA,
+                                                     const TI* __restrict__ B, int64_t ldb, int64_t sB, double beta,
+                                                     const TO* C, int64_t ldc, int64_t sC, TO* D, int64_t ldd,
+                                                     int64_t sD, int tilesM, int tilesN, int gn, OutMap om) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][2][kBM * kRow];  // [buf][hi/lo]
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][2][kBN * kRow];
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid & 1, wn = wid >> 1;
+
+  const int64_t nwg = (int64_t)tilesM * tilesN;
+  const int64_t w = blockIdx.x;
+  const int64_t q = nwg >> 3, r = nwg & 7, x = w & 7;
+  const int64_t L = x * q + (x < r ? x : r) + (w >> 3);
+  int64_t mt, nt;
+  if (gn <= 0) {
+    mt = L % tilesM;
+    nt = L / tilesM;
+  } else {
+    const int64_t g = L / ((int64_t)tilesM * gn);
+    const int64_t rem = L - g * (int64_t)tilesM * gn;
+    const int64_t wdt = (tilesN - g * gn < gn) ? tilesN - g * gn : gn;
+    mt = rem / wdt;
+    nt = g * gn + rem % wdt;
+  }
+  const int64_t m0 = mt * kBM, n0 = nt * kBN;
+  const int64_t bz = blockIdx.y;
+  A += bz * sA;
+  B += bz * sB;
+  if (C) C += bz * sC;
+  D += bz * sD;
+
+  // per-thread load coordinates
+  // row-contiguous pattern: rows 4*(tid&31)..+3, k 4*(tid>>5)..+3 (one load per k)
+  // k-contiguous pattern:  k 4*(tid&7)..+3, rows (tid>>3) + 32 j, j = 0..3
+  V4 ra[4], rb[4];
+  auto gload = [&](int64_t k0) {
+    if (!TA) {
+      const int64_t gm = m0 + 4 * (tid & 31);
+      const int n_ok = clamp4(M - gm);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t gk = k0 + 4 * (tid >> 5) + j;
+        ra[j] = load4<TI, VEC>(A + gm + lda * gk, n_ok == 4 && gk < K, gk < K ? n_ok : 0);
+      }
+    } else {
+      const int64_t gk = k0 + 4 * (tid & 7);
+      const int k_ok = clamp4(K - gk);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t gm = m0 + (tid >> 3) + 32 * j;
+        ra[j] = load4<TI, VEC>(A + gk + lda * gm, k_ok == 4 && gm < M, gm < M ? k_ok : 0);
+      }
+    }
+    if (TB) {
+      const int64_t gn_ = n0 + 4 * (tid & 31);
+      const int n_ok = clamp4(N - gn_);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t gk = k0 + 4 * (tid >> 5) + j;
+        rb[j] = load4<TI, VEC>(B + gn_ + ldb * gk, n_ok == 4 && gk < K, gk < K ? n_ok : 0);
+      }
+    } else {
+      const int64_t gk = k0 + 4 * (tid & 7);
+      const int k_ok = clamp4(K - gk);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t gcol = n0 + (tid >> 3) + 32 * j;
+        rb[j] = load4<TI, VEC>(B + gk + ldb * gcol, k_ok == 4 && gcol < N, gcol < N ? k_ok : 0);
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+    if (!TA) {
+      // ra[j].x[i] = A[row 4(tid&31)+i][k 4(tid>>5)+j]: gather the 4 k of each row
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const V4 t = {{ra[0].x[i], ra[1].x[i], ra[2].x[i], ra[3].x[i]}};
+        put4(As[buf][0], As[buf][1], (4 * (tid & 31) + i) * kRow + 4 * (tid >> 5), t);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) put4(As[buf][0], As[buf][1], ((tid >> 3) + 32 * j) * kRow + 4 * (tid & 7), ra[j]);
+    }
+    if (TB) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const V4 t = {{rb[0].x[i], rb[1].x[i], rb[2].x[i], rb[3].x[i]}};
+        put4(Bs[buf][0], Bs[buf][1], (4 * (tid & 31) + i) * kRow + 4 * (tid >> 5), t);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) put4(Bs[buf][0], Bs[buf][1], ((tid >> 3) + 32 * j) * kRow + 4 * (tid & 7), rb[j]);
+    }
+  };
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+
+  const int64_t nst = (K + kBK - 1) / kBK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  const int frow = lane & 31, fk = 8 * (lane >> 5);
+  for (int64_t s = 0; s < nst; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nst) gload((s + 1) * kBK);
+#pragma unroll
+    for (int kk = 0; kk < kBK / 16; ++kk) {
+      bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int off = (wm * 64 + i * 32 + frow) * kRow + kk * 16 + fk;
+        ah[i] = *reinterpret_cast<const bf16x8*>(&As[buf][0][off]);
+        al[i] = *reinterpret_cast<const bf16x8*>(&As[buf][1][off]);
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int off = (wn * 64 + j * 32 + frow) * kRow + kk * 16 + fk;
+        bh[j] = *reinterpret_cast<const bf16x8*>(&Bs[buf][0][off]);
+        bl[j] = *reinterpret_cast<const bf16x8*>(&Bs[buf][1][off]);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+        }
+    }
+    if (s + 1 < nst) sstore(buf ^ 1);
+    __syncthreads();
+  }
+
+  // epilogue: accumulator (i, j) element e of lane l is D[row][col] with
+  // row = m0 + wm*64 + 32 i + (l & 31), col = n0 + wn*64 + 32 j + 8 (e >> 2) + 4 (l >> 5) + (e & 3);
+  // output addresses as in the FP64 kernel (column-major with ldd, or the two-mode OutMap)
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int64_t row = m0 + wm * 64 + i * 32 + (lane & 31);
+    if (row >= M) continue;
+    const int64_t roff = om.rmod > 0 ? (row % om.rmod) * om.rs0 + (row / om.rmod) * om.rs1 : row;
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int64_t col = n0 + wn * 64 + j * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        if (col < N) {
+          int64_t coff;
+          if (om.rmod > 0) coff = om.cmod > 0 ? (col % om.cmod) * om.cs0 + (col / om.cmod) * om.cs1 : col * om.cs0;
+          else coff = col * ldd;
+          double val = alpha * (double)acc[i][j][e];
+          if (beta != 0.0) val += beta * (double)(om.rmod > 0 ? C[roff + coff] : C[row + ldc * col]);
+          D[roff + coff] = (TO)val;
+        }
+      }
+  }
+}
+
+// L2 budget of the B blocks of one raster group (bytes)
+static constexpr int64_t kGroupBytesBf3 = 1 << 20;
+
+template <typename TI>
+static bool aligned4(const TI* p, int64_t ld, int64_t stride) {
+  return ((uintptr_t)p % (4 * sizeof(TI))) == 0 && ld % 4 == 0 && stride % 4 == 0;
+}
+
+template <bool TA, bool TB, int TAG, typename TI, typename TO>
+static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda,
+                       int64_t sA, const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc,
+                       int64_t sC, TO* D, int64_t ldd, int64_t sD, int64_t batch, const OutMap& om) {
+  const int64_t tm = (M + kBM - 1) / kBM, tn = (N + kBN - 1) / kBN;
+  FH_CHECK(tm * tn < (int64_t)1 << 31 && batch < 65536, -3, "gemm grid too large");
+  dim3 grid((unsigned)(tm * tn), (unsigned)batch);
+  int gn = 0;
+  if (M > N) {
+    const int64_t per = std::max<int64_t>(1, K * kBN * (int64_t)sizeof(TI));
+    gn = (int)std::max<int64_t>(1, std::min<int64_t>(tn, kGroupBytesBf3 / per));
+  }
+  const bool vec = aligned4(A, lda, batch > 1 ? sA : 0) && aligned4(B, ldb, batch > 1 ? sB : 0);
+  if (vec)
+    hipLaunchKernelGGL((k_gemm_bf3<TA, TB, TAG, TI, TO, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B,
+                       ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, gn, om);
+  else
+    hipLaunchKernelGGL((k_gemm_bf3<TA, TB, TAG, TI, TO, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B,
+                       ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, gn, om);
+  FH_HIP(hipGetLastError());
+}
+
+template <typename TI, typename TO>
+void gemm_bf3(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda,
+              int64_t sA, const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc, int64_t sC,
+              TO* D, int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  FH_CHECK(K >= 0, -3, "gemm: negative K");
+  FH_CHECK(lda >= (ta ? (K > 0 ? K : 1) : M) && ldb >= (tb ? N : (K > 0 ? K : 1)) && (map || ldd >= M), -3,
+           "gemm: leading dimension too small");
+  FH_CHECK(beta == 0.0 || (C != nullptr && (map || ldc >= M)), -3, "gemm: C missing");
+  FH_CHECK(!map || (map->rmod > 0 && (beta == 0.0 || C == D)), -3, "gemm: mapped output needs rmod > 0 and C == D");
+  const OutMap om = map ? *map : OutMap{};
+  // instantiated variants: FP64 operands (any transposition, tag 0) for the contraction engine;
+  // FP32 operands, no transposition, for the PP ladder (tag 1, FP64 output) and the (T) W build
+  // (tag 2, FP32 output)
+  if constexpr (std::is_same<TI, double>::value) {
+    FH_CHECK(tag == 0, -3, "gemm_bf3: FP64 operands take tag 0");
+    if (ta) {
+      if (tb) launch_bf3<true, true, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om);
+      else launch_bf3<true, false, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om);
+    } else {
+      if (tb) launch_bf3<false, true, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om);
+      else launch_bf3<false, false, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om);
+    }
+  } else {
+    FH_CHECK(!ta && !tb, -3, "gemm_bf3: FP32 operands are not transposed");
+    constexpr int kTag = std::is_same<TO, double>::value ? 1 : 2;
+    FH_CHECK(tag == kTag, -3, "gemm_bf3: FP32 operands take tag 1 (FP64 output) or 2 (FP32 output)");
+    launch_bf3<false, false, kTag, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om);
+  }
+}
+
+template void gemm_bf3<float, float>(hipStream_t, bool, bool, int64_t, int64_t, int64_t, double, const float*, int64_t,
+                                     int64_t, const float*, int64_t, int64_t, double, const float*, int64_t, int64_t,
+                                     float*, int64_t, int64_t, int64_t, int, const OutMap*);
+template void gemm_bf3<float, double>(hipStream_t, bool, bool, int64_t, int64_t, int64_t, double, const float*,
+                                      int64_t, int64_t, const float*, int64_t, int64_t, double, const double*, int64_t,
+                                      int64_t, double*, int64_t, int64_t, int64_t, int, const OutMap*);
+template void gemm_bf3<double, double>(hipStream_t, bool, bool, int64_t, int64_t, int64_t, double, const double*,
+                                       int64_t, int64_t, const double*, int64_t, int64_t, double, const double*,
+                                       int64_t, int64_t, double*, int64_t, int64_t, int64_t, int, const OutMap*);
+
+}  // namespace fh

@@ -105,6 +105,37 @@ void gemm_f32_df(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, c
 void gemm_f32_ff(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const float* A, int64_t lda, int64_t sA,
                  const float* B, int64_t ldb, int64_t sB, double beta, const float* C, int64_t ldc, int64_t sC, float* D,
                  int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map);
+// FP32 operands and output, any transposition, C = alpha op(A) op(B) + beta C (tag 3)
+void gemm_f32(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const float* A,
+              int64_t lda, const float* B, int64_t ldb, double beta, float* C, int64_t ldc);
+// split-BF16x3 MFMA GEMM (three bf16 products per FP32 product, FP32 accumulation); TI in
+// {float, double}, TO in {float, double}; tags 0/1/2 give distinct kernel symbols as above
+template <typename TI, typename TO>
+void gemm_bf3(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda,
+              int64_t sA, const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc, int64_t sC,
+              TO* D, int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map);
+// ---------------------------------------------------------------- T2-guess network (fh_mlp.hip)
+struct MlpOffsets {
+  size_t W1, b1, W2, b2, W3, b3, w4, b4;
+};
+size_t mlp_nparams(int h);
+MlpOffsets mlp_offsets(int h);
+int64_t mlp_partial_elems(int h, int64_t B);  // floats of the bias-gradient scratch for batch B
+// forward of a batch X[6,B] -> activations A1..A3 [h,B] and outputs y[B]
+void mlp_forward(hipStream_t s, int h, const float* P, int64_t B, const float* X, float* A1, float* A2, float* A3,
+                 float* y);
+// gradient G of L = 1/(2B) sum (y - T)^2 after mlp_forward; dy[B], dA/dZ [h,B] scratch; lpart gets
+// *nlpart partial sums of B*L (grid-sized, at most 65536 doubles)
+void mlp_backward(hipStream_t s, int h, const float* P, int64_t B, const float* X, const float* T, const float* A1,
+                  const float* A2, const float* A3, const float* y, float* dy, float* dA, float* dZ, float* G,
+                  float* part, double* lpart, int64_t* nlpart);
+void mlp_adam(hipStream_t s, int64_t n, float* P, const float* G, float* m, float* v, float lr, int64_t t);
+void mlp_gather(hipStream_t s, int64_t B, int64_t off, const int64_t* perm, const float* X, const float* T, float* Xb,
+                float* Tb);
+void t2_features(hipStream_t s, int o, int v, int64_t e0, int64_t n, const double* t2, const double* eo,
+                 const double* ev, const double* dij, const float* xm, const float* xs, float* X);
+void t2_learned_apply(hipStream_t s, int o, int v, double ys, const float* Y, double* t2);
+
 // workspace (doubles) that gemm_f64 would use to split K for this shape (0: no split)
 int64_t gemm_splitk_elems(int64_t M, int64_t N, int64_t K, int64_t batch);
 // tag selects a distinct kernel symbol for profiling: 0 generic, 1 PP ladder, 2 (T) W build

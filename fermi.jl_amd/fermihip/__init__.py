@@ -24,13 +24,15 @@ import numpy as np
 
 __all__ = [
     "FermiException", "MOIntegrals", "Options", "RCCSDhip", "ijkhip", "RCCSD", "RCCSDpT", "Context",
-    "lib_path", "load_library", "EXPORTED_SYMBOLS",
+    "lib_path", "load_library", "EXPORTED_SYMBOLS", "Mlp", "PRECISIONS",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXPORTED_SYMBOLS = [
     "fh_default_opts", "fh_create", "fh_set_df", "fh_prepare", "fh_t2_guess", "fh_ccsd_energy",
     "fh_ccsd_residual", "fh_ccsd_solve", "fh_triples", "fh_stats", "fh_last_error", "fh_destroy",
+    "fh_mlp_create", "fh_mlp_num_params", "fh_mlp_get_params", "fh_mlp_set_params", "fh_mlp_forward",
+    "fh_mlp_loss_grad", "fh_mlp_train", "fh_mlp_destroy", "fh_set_t2_model",
 ]
 
 
@@ -45,7 +47,7 @@ class _Opts(ctypes.Structure):
         ("cc_diis_relax", ctypes.c_int), ("diis_prec", ctypes.c_int), ("cc_damp_ratio", ctypes.c_double),
         ("precision_override", ctypes.c_int), ("verbose", ctypes.c_int),
         ("precision_mode", ctypes.c_int), ("polish_iters", ctypes.c_int), ("mixed_e_switch", ctypes.c_double),
-        ("mixed_rms_switch", ctypes.c_double),
+        ("mixed_rms_switch", ctypes.c_double), ("bf3_e_switch", ctypes.c_double), ("bf3_rms_switch", ctypes.c_double),
     ]
 
 
@@ -53,7 +55,7 @@ class _Result(ctypes.Structure):
     _fields_ = [
         ("guess_energy", ctypes.c_double), ("correlation", ctypes.c_double), ("e_conv", ctypes.c_double),
         ("t_conv", ctypes.c_double), ("niter", ctypes.c_int), ("converged", ctypes.c_int),
-        ("niter_mixed", ctypes.c_int),
+        ("niter_mixed", ctypes.c_int), ("niter_bf3", ctypes.c_int),
     ]
 
 
@@ -89,8 +91,21 @@ def load_library():
     L.fh_last_error.restype = ctypes.c_char_p
     L.fh_destroy.argtypes = [ctypes.c_void_p]
     L.fh_destroy.restype = None
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.fh_mlp_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong]
+    L.fh_mlp_num_params.argtypes = [ctypes.c_void_p]
+    L.fh_mlp_num_params.restype = ctypes.c_size_t
+    L.fh_mlp_get_params.argtypes = [ctypes.c_void_p, fp]
+    L.fh_mlp_set_params.argtypes = [ctypes.c_void_p, fp]
+    L.fh_mlp_forward.argtypes = [ctypes.c_void_p, ctypes.c_longlong, fp, fp]
+    L.fh_mlp_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_longlong, fp, fp, dp, fp]
+    L.fh_mlp_train.argtypes = [ctypes.c_void_p, ctypes.c_longlong, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                               ctypes.c_ulonglong, dp]
+    L.fh_mlp_destroy.argtypes = [ctypes.c_void_p]
+    L.fh_mlp_destroy.restype = None
+    L.fh_set_t2_model.argtypes = [ctypes.c_void_p, ctypes.c_void_p, fp, fp, ctypes.c_double, dp]
     for name in EXPORTED_SYMBOLS:
-        if name not in ("fh_default_opts", "fh_last_error", "fh_destroy"):
+        if name not in ("fh_default_opts", "fh_last_error", "fh_destroy", "fh_mlp_destroy", "fh_mlp_num_params"):
             getattr(L, name).restype = ctypes.c_int
     _LIB = L
     return L
@@ -102,26 +117,43 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _samples(X):
+    """Sample matrix for the network: [n, 6] row-major float32 (= [6, n] column-major)."""
+    X = np.ascontiguousarray(np.asarray(X, dtype=np.float32))
+    if X.ndim != 2 or X.shape[1] != 6:
+        raise FermiException("network samples must be an [n, 6] array")
+    return X
+
+
 def _colmajor(a):
     """Julia arrays are column-major: hand the library a Fortran-ordered FP64 copy/view."""
     return np.asfortranarray(np.asarray(a, dtype=np.float64))
 
 
+PRECISIONS = {"double": 0, "mixed": 1, "bf16x3": 2}
+
+
 def Options(**kw):
     """Fermi option keys read on this path, with the reference defaults (Options.jl:62-102), plus
-    the mixed-precision keys of this backend: precision ("double" | "mixed": FP32 contractions
-    then FP64 polish), polish_iters (-1: polish to convergence), mixed_e_switch, mixed_rms_switch."""
+    the mixed-precision keys of this backend: precision ("double"; "mixed": FP32 contractions then
+    FP64 polish; "bf16x3": split-BF16x3 contractions, then FP32, then FP64 polish), polish_iters
+    (-1: polish to convergence), mixed_e_switch, mixed_rms_switch, bf3_e_switch, bf3_rms_switch."""
     o = dict(cc_max_iter=50, cc_e_conv=1e-10, cc_max_rms=1e-10, cc_diis=True, cc_ndiis=3, diis_start=3,
              cc_diis_relax=3, diis_prec="single", cc_damp_ratio=0.0, precision_override=False, verbose=False,
-             precision="double", polish_iters=-1, mixed_e_switch=1e-8, mixed_rms_switch=1e-7)
+             precision="double", polish_iters=-1, mixed_e_switch=1e-8, mixed_rms_switch=1e-7, bf3_e_switch=1e-6,
+             bf3_rms_switch=1e-5)
     for k, v in kw.items():
         if k not in o:
             raise FermiException(f"unknown option {k}")
         o[k] = v
     if o["diis_prec"] not in ("single", "double"):
         raise FermiException(f"invalid diis_prec {o['diis_prec']!r} (single|double)")
-    if o["precision"] not in ("double", "mixed"):
-        raise FermiException(f"invalid precision {o['precision']!r} (double|mixed)")
+    if o["precision"] not in PRECISIONS:
+        raise FermiException(f"invalid precision {o['precision']!r} (double|mixed|bf16x3)")
     return o
 
 
@@ -157,6 +189,7 @@ class Context:
         h = ctypes.c_void_p()
         self._check(self.L.fh_create(ctypes.byref(h), int(device)))
         self.h = h
+        self.device = int(device)
         self.o = self.v = self.naux = 0
 
     def _check(self, rc):
@@ -195,6 +228,22 @@ class Context:
         self._check(self.L.fh_t2_guess(self.h, kind, _ptr(T1), _ptr(T2)))
         return T1, T2
 
+    def set_t2_model(self, model, pair_dist=None):
+        """Attach a trained guess network (an Mlp, or a dict/npz with params, x_mean, x_scale,
+        y_scale, hidden) for guess(kind=1), with the occupied pair distances [o, o] in bohr."""
+        if isinstance(model, Mlp):
+            net, xm, xs, ys = model, model.x_mean, model.x_scale, model.y_scale
+        else:
+            net = Mlp(hidden=int(model["hidden"]), device=self.device)
+            net.set_params(np.asarray(model["params"], dtype=np.float32))
+            xm, xs, ys = model["x_mean"], model["x_scale"], float(model["y_scale"])
+        xm = np.ascontiguousarray(np.asarray(xm, dtype=np.float32))
+        xs = np.ascontiguousarray(np.asarray(xs, dtype=np.float32))
+        d = None if pair_dist is None else _colmajor(pair_dist)
+        if d is not None and d.shape != (self.o, self.o):
+            raise FermiException("pair_dist must be [o, o]")
+        self._check(self.L.fh_set_t2_model(self.h, net.h, _fptr(xm), _fptr(xs), float(ys), _ptr(d)))
+
     def energy(self, T1=None, T2=None):
         E = ctypes.c_double()
         if T1 is None:
@@ -221,10 +270,12 @@ class Context:
         c.cc_diis_relax = int(opts["cc_diis_relax"]); c.diis_prec = 1 if opts["diis_prec"] == "single" else 0
         c.cc_damp_ratio = float(opts["cc_damp_ratio"]); c.precision_override = int(bool(opts["precision_override"]))
         c.verbose = int(bool(opts["verbose"]))
-        c.precision_mode = 1 if opts["precision"] == "mixed" else 0
+        c.precision_mode = PRECISIONS[opts["precision"]]
         c.polish_iters = int(opts["polish_iters"])
         c.mixed_e_switch = float(opts["mixed_e_switch"])
         c.mixed_rms_switch = float(opts["mixed_rms_switch"])
+        c.bf3_e_switch = float(opts["bf3_e_switch"])
+        c.bf3_rms_switch = float(opts["bf3_rms_switch"])
         r = _Result()
         if T1 is None:
             self._check(self.L.fh_ccsd_solve(self.h, ctypes.byref(c), None, None, ctypes.byref(r)))
@@ -318,3 +369,75 @@ def RCCSDpT(ccsd, moints, alg=None, device=0):
     ctx = _ctx_for(moints, device)
     Et = ctx.triples(ccsd.T1, ccsd.T2)
     return RCCSDpTResult(ccsd, Et + ccsd.energy, Et)
+
+
+class Mlp:
+    """The learned T2-guess network on the GPU (fh_mlp_*): 6 features -> 3 tanh layers -> 1.
+    x_mean, x_scale, y_scale hold the feature/target normalisation used with a context."""
+
+    def __init__(self, hidden=128, device=0, seed=0):
+        self.L = load_library()
+        self.hidden = int(hidden)
+        self.device = int(device)
+        h = ctypes.c_void_p()
+        self._check(self.L.fh_mlp_create(ctypes.byref(h), self.device, self.hidden, int(seed)))
+        self.h = h.value
+        self.x_mean = np.zeros(6, np.float32)
+        self.x_scale = np.ones(6, np.float32)
+        self.y_scale = 1.0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FermiException(f"libfermihip error {rc}: {self.L.fh_last_error().decode()}")
+
+    @property
+    def num_params(self):
+        return int(self.L.fh_mlp_num_params(self.h))
+
+    def get_params(self):
+        p = np.zeros(self.num_params, np.float32)
+        self._check(self.L.fh_mlp_get_params(self.h, _fptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(np.asarray(p, dtype=np.float32))
+        if p.shape != (self.num_params,):
+            raise FermiException(f"expected {self.num_params} parameters")
+        self._check(self.L.fh_mlp_set_params(self.h, _fptr(p)))
+
+    def forward(self, X):
+        X = _samples(X)
+        Y = np.zeros(X.shape[0], np.float32)
+        self._check(self.L.fh_mlp_forward(self.h, X.shape[0], _fptr(X), _fptr(Y)))
+        return Y
+
+    def loss_grad(self, X, T):
+        X = _samples(X)
+        T = np.ascontiguousarray(np.asarray(T, dtype=np.float32))
+        g = np.zeros(self.num_params, np.float32)
+        loss = ctypes.c_double()
+        self._check(self.L.fh_mlp_loss_grad(self.h, X.shape[0], _fptr(X), _fptr(T), ctypes.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def train(self, X, T, epochs=10, batch=8192, lr=1e-3, seed=0):
+        X = _samples(X)
+        T = np.ascontiguousarray(np.asarray(T, dtype=np.float32))
+        hist = np.zeros(max(epochs, 1))
+        self._check(self.L.fh_mlp_train(self.h, X.shape[0], _fptr(X), _fptr(T), int(epochs), int(batch), float(lr),
+                                        int(seed), _ptr(hist)))
+        return hist[:epochs]
+
+    def state(self):
+        return {"params": self.get_params(), "hidden": self.hidden, "x_mean": self.x_mean, "x_scale": self.x_scale,
+                "y_scale": self.y_scale}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fh_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

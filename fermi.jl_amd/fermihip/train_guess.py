@@ -1,0 +1,153 @@
+"""Train the learned T2 initial guess on converged amplitudes that the engine itself generates.
+
+    python fermi.jl_amd/fermihip/train_guess.py --out model.npz [--per-size 40] [--epochs 30]
+
+1. Training set: seeded synthetic (H2O)n lattice clusters, n = 2..6 (``synth.water_cluster_like``),
+   ``--per-size`` clusters of each size.  For each, the GPU engine computes the MP2 guess and
+   the converged FP64 CCSD amplitudes (mixed-precision iterations with FP64 polish to the
+   reference's 1e-10 criteria).
+2. Samples: per cluster up to ``--samples`` amplitudes, half drawn uniformly and half with
+   probability proportional to |T2_MP2|; features (T2_MP2, eps_i, eps_j, eps_a, eps_b, d_ij),
+   target T2_CCSD - T2_MP2.  Features and target are standardised over the training set.
+3. The network (3 x 128 tanh) is trained on the GPU by libfermihip's own forward/backward/Adam
+   kernels (``fh_mlp_train``).
+4. Evaluation on held-out clusters: CCSD iterations and energy from the MP2 guess and from the
+   learned guess (the converged energies must agree to 1e-8 Eh).
+
+The model file holds params, hidden, x_mean, x_scale, y_scale and a JSON ``info`` record.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if os.path.dirname(HERE) not in sys.path:
+    sys.path.insert(0, os.path.dirname(HERE))
+
+import fermihip  # noqa: E402
+from fermihip import synth  # noqa: E402
+
+DEFAULT_MODEL = os.path.join(HERE, "t2_guess_water.npz")
+
+
+def cluster_features(T2mp2, eo, ev, dij):
+    """Raw feature rows [n, 6] of every amplitude, column-major element order (as on the GPU)."""
+    o, _, v, _ = T2mp2.shape
+    cols = [T2mp2.ravel(order="F")]
+    e = np.arange(o * o * v * v)
+    i = e % o
+    j = (e // o) % o
+    a = (e // (o * o)) % v
+    b = e // (o * o * v)
+    cols += [eo[i], eo[j], ev[a], ev[b], dij[i, j]]
+    return np.stack(cols, axis=1)
+
+
+def converged_cluster(nw, seed, opts):
+    eo, ev, Boo, Bov, Bvv, so, pos = synth.water_cluster_like(nw, seed)
+    ctx = fermihip.Context(0)
+    ctx.set_df(eo, ev, Boo, Bov, Bvv)
+    T1, T2 = ctx.guess()
+    r, _, t2 = ctx.solve(opts, T1, T2)
+    ctx.close()
+    return eo, ev, synth.pair_distances(so, pos), T2, t2, r
+
+
+def load_model(path=DEFAULT_MODEL):
+    """The committed model as a dict for Context.set_t2_model (None when the file is absent)."""
+    if not os.path.exists(path):
+        return None
+    z = np.load(path, allow_pickle=False)
+    return {k: z[k] for k in ("params", "hidden", "x_mean", "x_scale", "y_scale")}
+
+
+def evaluate(model, nw, seed, precision="double"):
+    """Iterations / energies / wall time from the MP2 guess and from the learned guess."""
+    eo, ev, Boo, Bov, Bvv, so, pos = synth.water_cluster_like(nw, seed)
+    ctx = fermihip.Context(0)
+    ctx.set_df(eo, ev, Boo, Bov, Bvv)
+    out = {"waters": nw, "seed": seed, "precision": precision}
+    for kind, name in ((0, "mp2"), (1, "learned")):
+        if kind == 1:
+            ctx.set_t2_model(model, synth.pair_distances(so, pos))
+        t0 = time.perf_counter()
+        ctx.guess(kind=kind, copy_out=False)
+        r, _, _ = ctx.solve(fermihip.Options(precision=precision), copy_out=False)
+        out[name] = {"niter": r.niter, "energy": r.correlation, "guess_energy": r.guess_energy,
+                     "seconds": time.perf_counter() - t0, "converged": bool(r.converged)}
+    out["delta_E"] = out["learned"]["energy"] - out["mp2"]["energy"]
+    ctx.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--out", default=DEFAULT_MODEL)
+    ap.add_argument("--per-size", type=int, default=40, help="clusters per size n = 2..6")
+    ap.add_argument("--samples", type=int, default=30000, help="amplitudes sampled per cluster")
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--hidden", type=int, default=128)
+    ap.add_argument("--eval", default="4:7,10:0", help="held-out clusters waters:seed,...")
+    args = ap.parse_args()
+
+    rng = np.random.default_rng(12345)
+    opts = fermihip.Options(precision="mixed")
+    Xs, Ys, nsys, t0 = [], [], 0, time.perf_counter()
+    for nw in range(2, 7):
+        for k in range(args.per_size):
+            seed = 1000 * nw + k
+            eo, ev, dij, T2m, T2c, r = converged_cluster(nw, seed, opts)
+            if not r.converged:
+                print(f"train_guess: skip (H2O){nw} seed {seed}: not converged", flush=True)
+                continue
+            F = cluster_features(T2m, eo, ev, dij)
+            dT = (T2c - T2m).ravel(order="F")
+            n = F.shape[0]
+            m = min(n, args.samples)
+            w = np.abs(F[:, 0]) + 1e-30
+            pick = np.concatenate([rng.choice(n, m // 2, replace=False),
+                                   rng.choice(n, m - m // 2, replace=False, p=w / w.sum())])
+            Xs.append(F[pick])
+            Ys.append(dT[pick])
+            nsys += 1
+        print(f"train_guess: (H2O){nw} done, {nsys} clusters, {time.perf_counter() - t0:.1f} s", flush=True)
+    X = np.concatenate(Xs)
+    Y = np.concatenate(Ys)
+    x_mean = X.mean(0)
+    x_scale = 1.0 / np.maximum(X.std(0), 1e-12)
+    y_scale = float(max(Y.std(), 1e-30))
+    Xn = ((X - x_mean) * x_scale).astype(np.float32)
+    Yn = (Y / y_scale).astype(np.float32)
+    print(f"train_guess: {X.shape[0]} samples from {nsys} clusters; target std {y_scale:.3e}", flush=True)
+
+    net = fermihip.Mlp(hidden=args.hidden, seed=7)
+    net.x_mean, net.x_scale, net.y_scale = x_mean.astype(np.float32), x_scale.astype(np.float32), y_scale
+    t1 = time.perf_counter()
+    hist = net.train(Xn, Yn, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=11)
+    t_train = time.perf_counter() - t1
+    print(f"train_guess: {args.epochs} epochs in {t_train:.1f} s; loss {hist[0]:.4e} -> {hist[-1]:.4e}", flush=True)
+    model = net.state()
+    evals = []
+    for item in filter(None, args.eval.split(",")):
+        nw, seed = (int(t) for t in item.split(":"))
+        ev = evaluate(model, nw, seed)
+        evals.append(ev)
+        print(f"train_guess: eval (H2O){nw} seed {seed}: niter MP2 {ev['mp2']['niter']} -> learned "
+              f"{ev['learned']['niter']}, dE {ev['delta_E']:.2e}", flush=True)
+    info = {"clusters": nsys, "sizes": "2-6 waters", "samples": int(X.shape[0]), "epochs": args.epochs,
+            "batch": args.batch, "lr": args.lr, "loss_history": [float(h) for h in hist],
+            "train_seconds": t_train, "eval": evals}
+    np.savez(args.out, params=model["params"], hidden=np.int64(args.hidden), x_mean=model["x_mean"],
+             x_scale=model["x_scale"], y_scale=np.float64(y_scale), info=np.array(json.dumps(info)))
+    print(json.dumps({"model": args.out, **{k: info[k] for k in ("clusters", "samples", "epochs")},
+                      "final_loss": info["loss_history"][-1], "eval": evals}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

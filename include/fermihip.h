@@ -43,13 +43,17 @@ typedef struct fh_cc_opts {
   /* Mixed precision (no counterpart in the reference, whose only knob is all-FP32 CC):
    * precision_mode 0 = FP64 contractions throughout; 1 = the residual's contractions run as
    * FP32 MFMA with FP32 accumulation (amplitudes, denominators, DIIS and energies stay FP64)
-   * until |dE| < mixed_e_switch and rms < mixed_rms_switch, then FP64 polish iterations follow:
+   * until |dE| < mixed_e_switch and rms < mixed_rms_switch, then FP64 polish iterations follow;
+   * 2 = the o^3v^3 and o^2v^4 contractions first run as split-BF16x3 MFMA (three bf16 products
+   * per FP32 product) until |dE| < bf3_e_switch and rms < bf3_rms_switch, then as in mode 1.
    * polish_iters < 0 iterates in FP64 to the cc_e_conv / cc_max_rms criteria, polish_iters >= 0
-   * stops after that many FP64 iterations. */
+   * stops after that many FP64 iterations.  Every precision switch restarts the DIIS subspace. */
   int precision_mode;
   int polish_iters;
-  double mixed_e_switch;
-  double mixed_rms_switch;
+  double mixed_e_switch;    /* default 1e-8 */
+  double mixed_rms_switch;  /* default 1e-7 */
+  double bf3_e_switch;      /* default 1e-6 */
+  double bf3_rms_switch;    /* default 1e-5 */
 } fh_cc_opts;
 
 /* Result of fh_ccsd_solve: the scalar fields of Fermi's RCCSD{T} (RCCSD.jl:61-69),
@@ -61,7 +65,8 @@ typedef struct fh_cc_result {
   double t_conv;          /* last max(rms(dT1), rms(dT2)) */
   int niter;
   int converged;
-  int niter_mixed;        /* iterations run with FP32 contractions (precision_mode 1) */
+  int niter_mixed;        /* iterations run with reduced-precision contractions (FP32 or BF16x3) */
+  int niter_bf3;          /* of these, iterations run with split-BF16x3 contractions */
 } fh_cc_result;
 
 /* Fill opts with Fermi's defaults. */
@@ -83,6 +88,10 @@ int fh_set_df(fh_ctx* ctx, int o, int v, int naux, const double* eps_o, const do
 int fh_prepare(fh_ctx* ctx);
 
 /* Initial amplitudes.  kind 0: MP2 guess T1 = 0, T2 = (ia|jb)/D2 (RCCSDa.jl:46-90).
+ * kind 1: learned guess (no counterpart in the reference): the MP2 guess plus the correction
+ * predicted by the network attached with fh_set_t2_model, symmetrised in (ij)(ab); T1 = 0.
+ * The CC iterations converge to the same amplitudes from either guess; the MP2 guess stays the
+ * fallback (kind 1 fails with -6 when no network is attached).
  * T1[o,v], T2[o,o,v,v] are host outputs (either may be NULL); the guess is also kept on
  * the device as the starting point of fh_ccsd_solve with NULL amplitudes. */
 int fh_t2_guess(fh_ctx* ctx, int kind, double* T1, double* T2);
@@ -100,12 +109,37 @@ int fh_ccsd_residual(fh_ctx* ctx, const double* T1, const double* T2, double* ne
 int fh_ccsd_solve(fh_ctx* ctx, const fh_cc_opts* opts, double* T1, double* T2, fh_cc_result* res);
 
 /* (T) correction RCCSDpT(ccsd, moints, ::ijk), ijk.jl:20-150.  prec_mode 0 = FP64; 1 = the W
- * build runs as FP32 MFMA with FP32 accumulation, V, denominators and the energy sum stay FP64.
+ * build runs as FP32 MFMA with FP32 accumulation; 2 = the W build runs as split-BF16x3 MFMA
+ * with FP32 accumulation.  V, denominators and the energy sum stay FP64 in every mode.
  * T1/T2 NULL: use the device-resident amplitudes of the last solve. */
 int fh_triples(fh_ctx* ctx, int prec_mode, const double* T1, const double* T2, double* Et);
 
 /* Per-phase timings, flop counts and the iteration table as JSON (NUL-terminated). */
 int fh_stats(fh_ctx* ctx, char* json, size_t cap);
+
+/* ---- Learned T2 guess network (no counterpart in the reference).
+ * A per-amplitude MLP: 6 features -> 3 tanh layers of width `hidden` -> 1, FP32 on FP32 MFMA.
+ * Features of T2[i,j,a,b]: (T2_MP2[i,j,a,b], eps_i, eps_j, eps_a, eps_b, d_ij), standardised as
+ * (x - x_mean) * x_scale; the output times y_scale is T2_CCSD - T2_MP2.
+ * Parameters (FP32, flat): W1[h,6] b1[h] W2[h,h] b2[h] W3[h,h] b3[h] w4[h] b4, W column-major
+ * [out,in].  Sample matrices X are [6,n] column-major (the 6 features of a sample contiguous). */
+typedef struct fh_mlp fh_mlp;
+int fh_mlp_create(fh_mlp** net, int device, int hidden, unsigned long long seed);  /* Glorot init */
+size_t fh_mlp_num_params(const fh_mlp* net);
+int fh_mlp_get_params(fh_mlp* net, float* params);
+int fh_mlp_set_params(fh_mlp* net, const float* params);  /* also resets the optimiser state */
+int fh_mlp_forward(fh_mlp* net, long long n, const float* X, float* Y);
+/* loss L = 1/(2n) sum (y - T)^2 of one batch and its gradient (grad may be NULL) */
+int fh_mlp_loss_grad(fh_mlp* net, long long n, const float* X, const float* T, double* loss, float* grad);
+/* Adam (beta 0.9/0.999, eps 1e-8) over shuffled minibatches; epoch_loss[epochs] (may be NULL) */
+int fh_mlp_train(fh_mlp* net, long long n, const float* X, const float* T, int epochs, int batch, float lr,
+                 unsigned long long seed, double* epoch_loss);
+void fh_mlp_destroy(fh_mlp* net);
+/* Attach a copy of the network to a context for fh_t2_guess(kind 1), with the normalisation
+ * x_mean[6], x_scale[6], y_scale and the occupied pair distances pair_dist[o,o] (bohr; NULL: 0).
+ * Per molecule: fh_set_df detaches it. */
+int fh_set_t2_model(fh_ctx* ctx, const fh_mlp* net, const float* x_mean, const float* x_scale, double y_scale,
+                    const double* pair_dist);
 
 /* Message of the last failed call on this thread. */
 const char* fh_last_error(void);

@@ -1,0 +1,107 @@
+"""GPU parity of the learned T2-guess network (fh_mlp_* through the C ABI) against the CPU
+oracle (oracle/mlp.py): forward pass, loss gradient, training, and the guess assembly."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _rand_net(h, seed, scale=0.3):
+    return (scale * np.random.default_rng(seed).standard_normal(M.nparams(h))).astype(np.float32)
+
+
+@pytest.mark.parametrize("h,n", [(128, 1000), (64, 4097), (128, 70001)])
+def test_forward_matches_oracle(h, n):
+    say(f"mlp forward h={h} n={n}")
+    import fermihip
+    net = fermihip.Mlp(hidden=h)
+    p = _rand_net(h, 1)
+    net.set_params(p)
+    assert np.array_equal(net.get_params(), p)
+    X = np.random.default_rng(2).standard_normal((n, 6)).astype(np.float32)
+    y = net.forward(X)
+    ref, _ = M.forward(p.astype(np.float64), h, X.astype(np.float64))
+    # FP32 MFMA layers: relative to the output scale
+    assert np.max(np.abs(y - ref)) < 2e-5 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("h,n", [(128, 2000), (32, 513)])
+def test_gradient_matches_oracle(h, n):
+    say(f"mlp gradient h={h} n={n}")
+    import fermihip
+    net = fermihip.Mlp(hidden=h)
+    p = _rand_net(h, 3)
+    net.set_params(p)
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((n, 6)).astype(np.float32)
+    T = rng.standard_normal(n).astype(np.float32)
+    L, g = net.loss_grad(X, T)
+    Lr, gr = M.loss_grad(p.astype(np.float64), h, X.astype(np.float64), T.astype(np.float64))
+    assert abs(L - Lr) < 1e-5 * abs(Lr)
+    assert np.max(np.abs(g - gr)) < 1e-4 * np.max(np.abs(gr))
+    o = M.offsets(h)
+    for k in ("W1", "b1", "W2", "b2", "W3", "b3", "w4"):        # every block, not only the largest
+        blk = slice(o[k], o[k] + (6 * h if k == "W1" else h * h if k in ("W2", "W3") else h))
+        assert np.max(np.abs(g[blk] - gr[blk])) < 1e-4 * np.max(np.abs(gr[blk])) + 1e-9, k
+
+
+def test_training_reduces_loss():
+    say("mlp training")
+    import fermihip
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((20000, 6)).astype(np.float32)
+    T = (np.sin(X[:, 0]) + 0.5 * X[:, 1] * X[:, 2] - 0.3 * X[:, 5]).astype(np.float32)
+    net = fermihip.Mlp(hidden=64, seed=1)
+    p0 = net.get_params()
+    L0 = M.loss(p0.astype(np.float64), 64, X, T)
+    hist = net.train(X, T, epochs=15, batch=1024, lr=3e-3, seed=2)
+    L1 = M.loss(net.get_params().astype(np.float64), 64, X, T)
+    say(f"  loss {L0:.3e} -> {L1:.3e} (epochs {hist[0]:.3e} .. {hist[-1]:.3e})")
+    assert np.all(np.isfinite(hist)) and L1 < 0.05 * L0
+    assert abs(hist[-1] - L1) < 0.5 * L1 + 1e-3
+
+
+def test_learned_guess_assembly():
+    """guess(kind=1) = MP2 + y_scale * (Y[ijab] + Y[jiba]) / 2 with Y the network output on the
+    standardised features, and the CC fixed point does not depend on the guess."""
+    say("learned guess assembly (H2O)2")
+    import fermihip
+    from fermihip import synth
+    eo, ev, Boo, Bov, Bvv, so, pos = synth.water_cluster_like(2, 3)
+    d = synth.pair_distances(so, pos)
+    c = fermihip.Context(0)
+    c.set_df(eo, ev, Boo, Bov, Bvv)
+    T1m, T2m = c.guess(kind=0)
+    h = 32
+    net = fermihip.Mlp(hidden=h)
+    p = _rand_net(h, 6, scale=0.2)
+    net.set_params(p)
+    net.x_mean = np.array([0.0, -0.7, -0.7, 1.5, 1.5, 3.0], np.float32)
+    net.x_scale = np.array([100.0, 2.0, 2.0, 0.8, 0.8, 0.3], np.float32)
+    net.y_scale = 1e-3
+    c.set_t2_model(net, d)
+    T1l, T2l = c.guess(kind=1)
+    F = M.features(T2m, eo, ev, d, net.x_mean, net.x_scale)
+    y, _ = M.forward(p.astype(np.float64), h, F.astype(np.float32).astype(np.float64))
+    o, v = T2m.shape[0], T2m.shape[2]
+    Y = y.reshape(v, v, o, o).transpose(3, 2, 1, 0)
+    ref = T2m + net.y_scale * 0.5 * (Y + np.transpose(Y, (1, 0, 3, 2)))
+    assert np.array_equal(T1l, T1m)
+    assert np.max(np.abs(T2l - ref)) < 1e-5 * net.y_scale * max(1.0, np.max(np.abs(Y)))
+    asym = lambda t: np.max(np.abs(t - np.transpose(t, (1, 0, 3, 2))))
+    assert asym(T2l) <= asym(T2m) + 1e-15
+    r0, _, _ = c.solve(fermihip.Options(), T1m, T2m)
+    r1, _, _ = c.solve(fermihip.Options(), T1l, T2l)
+    assert r0.converged and r1.converged and abs(r0.correlation - r1.correlation) < 1e-8
+    c.set_df(eo, ev, Boo, Bov, Bvv)          # a new molecule detaches the model
+    with pytest.raises(fermihip.FermiException, match="fh_set_t2_model"):
+        c.guess(kind=1)

@@ -172,3 +172,40 @@ def test_mixed_precision_matches_fp64(cfg, seed):
     e64 = c.triples(a1, a2, prec_mode=0)
     e32 = c.triples(a1, a2, prec_mode=1)
     assert abs(e32 - e64) < 1e-6 * abs(e64)
+
+
+@pytest.mark.parametrize("o,v,Q,seed,t1s", [(2, 3, 6, 0, 0.1), (5, 19, 84, 2, 0.05), (7, 12, 30, 4, 0.2)])
+def test_triples_bf16x3_and_fp32(o, v, Q, seed, t1s):
+    """(T) with the W build in FP32 MFMA and in split-BF16x3 MFMA against the FP64 oracle.  A
+    bf16x3 product carries a relative error of about 2^-16 (the dropped lo*lo term and the
+    rounding of lo), FP32 about 2^-23; the energy bounds allow for that without cancellation."""
+    say(f"triples fp32/bf16x3 o={o} v={v} Q={Q} seed={seed}")
+    eo, ev, Boo, Bov, Bvv = random_df(o, v, Q, seed)
+    m = R.build_moints(Boo, Bov, Bvv)
+    T1, T2 = random_amplitudes(o, v, seed, s1=t1s)
+    c = ctx_for(eo, ev, Boo, Bov, Bvv)
+    e_ref = TT.rccsdpt_ijk(T1, T2, m, eo, ev)
+    e32 = c.triples(T1, T2, prec_mode=1)
+    e16 = c.triples(T1, T2, prec_mode=2)
+    say(f"  rel err fp32 {abs(e32 - e_ref) / abs(e_ref):.2e}  bf16x3 {abs(e16 - e_ref) / abs(e_ref):.2e}")
+    assert abs(e32 - e_ref) < 1e-6 * abs(e_ref)
+    assert abs(e16 - e_ref) < 3e-5 * abs(e_ref)
+
+
+@pytest.mark.parametrize("cfg,seed", [("water1", 0), ("water4", 1)])
+def test_bf16x3_cascade_matches_fp64(cfg, seed):
+    """split-BF16x3 -> FP32 -> FP64 contraction cascade reaches the FP64 fixed point (1e-9 Eh)."""
+    say(f"bf16x3 cascade {cfg}")
+    import fermihip
+    from fermihip import synth
+    o, v, Q = synth.SHAPES[cfg]
+    eo, ev, Boo, Bov, Bvv = synth.molecular_like(o, v, Q, seed)
+    c = ctx_for(eo, ev, Boo, Bov, Bvv)
+    T1, T2 = c.guess()
+    r64, a1, a2 = c.solve(fermihip.Options(), T1, T2)
+    r16, b1, b2 = c.solve(fermihip.Options(precision="bf16x3"), T1, T2)
+    say(f"  niter fp64 {r64.niter}; cascade {r16.niter} (bf16x3 {r16.niter_bf3}, reduced {r16.niter_mixed}); "
+        f"dE {r16.correlation - r64.correlation:.2e}")
+    assert r64.converged == 1 and r16.converged == 1
+    assert r16.niter_bf3 > 0 and r16.niter_mixed >= r16.niter_bf3 and r16.niter > r16.niter_mixed
+    assert abs(r16.correlation - r64.correlation) < 1e-9
