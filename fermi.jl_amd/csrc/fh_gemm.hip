@@ -28,7 +28,7 @@ namespace fh {
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-template <int BM, int BN, int BK, bool TA, bool TB, int OCC, int TAG, bool F32, typename TI, typename TO>
+template <int BM, int BN, int BK, bool TA, bool TB, int OCC, int TAG, bool F32, typename TI, typename TO, bool LM>
 __global__ __launch_bounds__(256, OCC) void k_gemm(
     int64_t M, int64_t N, int64_t K, double alpha, const TI* __restrict__ A, int64_t lda, int64_t sA,
     const TI* __restrict__ B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc, int64_t sC, TO* D,
@@ -141,38 +141,57 @@ __global__ __launch_bounds__(256, OCC) void k_gemm(
       for (int i = 0; i < MB; ++i)
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-          if constexpr (F32) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
-          else acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+          // LM: B fragment as the first MFMA operand, so the accumulator lanes run along M
+          if constexpr (F32) {
+            if constexpr (LM) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+            else acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+          } else {
+            if constexpr (LM) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+            else acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+          }
         }
     }
     if (s + 1 < nst) sstore(buf ^ 1);
     __syncthreads();
   }
 
-  // epilogue: lane holds column lane&15 and rows (lane>>4)+4r (FP64 MFMA) or 4(lane>>4)+r
-  // (FP32 MFMA) of each 16x16 block.  Output address: column-major with ldd, or (om.rmod > 0)
-  // the two-mode map row r -> (r % rmod)*rs0 + (r / rmod)*rs1, column c -> (c % cmod)*cs0 +
-  // (c / cmod)*cs1 (cmod = 0: c*cs0), which lets a GEMM accumulate into a permuted tensor.
+  // epilogue: the accumulator of block (i, j) holds, in lane l and register r, the element at
+  // (lane index = l & 15, register index = (l >> 4) + 4 r for FP64 MFMA or 4 (l >> 4) + r for FP32
+  // MFMA) of the 16x16 block; the lane index runs along M when LM (rows of D), else along N.
+  // Output address: column-major with ldd, or (om.rmod > 0) the two-mode map
+  // row r -> (r % rmod)*rs0 + (r / rmod)*rs1, column c -> (c % cmod)*cs0 + (c / cmod)*cs1
+  // (cmod = 0: c*cs0), which lets a GEMM accumulate into a permuted tensor.  LM is chosen so
+  // that the lanes of one store run along the unit-stride axis of the output.
 #pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    const int64_t col = n0 + wn * WN + j * 16 + (lane & 15);
-    if (col >= N) continue;
-    int64_t coff;
-    if (om.rmod > 0) coff = om.cmod > 0 ? (col % om.cmod) * om.cs0 + (col / om.cmod) * om.cs1 : col * om.cs0;
-    else coff = col * ldd;
+  for (int i = 0; i < MB; ++i)
 #pragma unroll
-    for (int i = 0; i < MB; ++i)
+    for (int j = 0; j < NB; ++j)
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int64_t row = m0 + wm * WM + i * 16 + (F32 ? 4 * (lane >> 4) + rr : (lane >> 4) + 4 * rr);
-        if (row < M) {
-          const int64_t roff = om.rmod > 0 ? (row % om.rmod) * om.rs0 + (row / om.rmod) * om.rs1 : row;
+        const int li = lane & 15, ri = F32 ? 4 * (lane >> 4) + rr : (lane >> 4) + 4 * rr;
+        const int64_t row = m0 + wm * WM + i * 16 + (LM ? li : ri);
+        const int64_t col = n0 + wn * WN + j * 16 + (LM ? ri : li);
+        if (row < M && col < N) {
+          int64_t off;
+          if (om.rmod > 0) {
+            const int64_t roff = (row % om.rmod) * om.rs0 + (row / om.rmod) * om.rs1;
+            const int64_t coff = om.cmod > 0 ? (col % om.cmod) * om.cs0 + (col / om.cmod) * om.cs1 : col * om.cs0;
+            off = roff + coff;
+          } else {
+            off = row + ldd * col;
+          }
           double val = alpha * (double)acc[i][j][rr];
-          if (beta != 0.0) val += beta * (double)(om.rmod > 0 ? C[roff + coff] : C[row + ldc * col]);
-          D[roff + coff] = (TO)val;
+          if (beta != 0.0) val += beta * (double)(om.rmod > 0 ? C[off] : C[row + ldc * col]);
+          D[off] = (TO)val;
         }
       }
-  }
+}
+
+// lanes along M unless only the output's column axis has unit stride
+static bool lanes_along_m(const OutMap& om) {
+  if (om.rmod <= 0) return true;
+  if (om.rs0 == 1) return true;
+  return om.cs0 != 1;
 }
 
 // L2 budget of the B blocks of one raster group (bytes)
@@ -192,8 +211,14 @@ static void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha,
     const int64_t per = std::max<int64_t>(1, kl * BN * (int64_t)sizeof(TI));
     gn = (int)std::max<int64_t>(1, std::min<int64_t>(tn, kGroupBytes / per));
   }
-  hipLaunchKernelGGL((k_gemm<BM, BN, BK, TA, TB, (BM * BN > 8192 ? 1 : 4), TAG, F32, TI, TO>), grid, dim3(256), 0, s, M,
-                     N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, gn, om);
+  constexpr int OCC = BM * BN > 8192 ? 1 : 4;
+  // lane orientation along N only for mapped outputs (the (T) W build, tag 2)
+  if (TAG == 2 && !lanes_along_m(om))
+    hipLaunchKernelGGL((k_gemm<BM, BN, BK, TA, TB, OCC, TAG, F32, TI, TO, false>), grid, dim3(256), 0, s, M, N, K, alpha,
+                       A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, gn, om);
+  else
+    hipLaunchKernelGGL((k_gemm<BM, BN, BK, TA, TB, OCC, TAG, F32, TI, TO, true>), grid, dim3(256), 0, s, M, N, K, alpha,
+                       A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, gn, om);
   FH_HIP(hipGetLastError());
 }
 

@@ -92,7 +92,7 @@ __device__ inline V4 load4(const TI* p, bool full, int n_ok) {
 // Operand tile layout in LDS: plane[row][k] (k fastest, kRow bf16 per row), row = m for A, n for B.
 // "row-contiguous" operands (A with TA = false, B with TB = true) are read 4 rows x 4 k per thread
 // and transposed in registers; "k-contiguous" operands are read 4 k of one row per load.
-template <bool TA, bool TB, int TAG, typename TI, typename TO, bool VEC>
+template <bool TA, bool TB, int TAG, typename TI, typename TO, bool VEC, bool LM>
 __global__ __launch_bounds__(256, 2) void k_gemm_bf3(int64_t M, int64_t N, int64_t K, double alpha,
                                                      const TI* __restrict__ A, int64_t lda, int64_t sA,
                                                      const TI* __restrict__ B, int64_t ldb, int64_t sB, double beta,
@@ -225,38 +225,50 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf3(int64_t M, int64_t N, int64
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+          // LM: B fragment as the first MFMA operand, so the accumulator lanes run along M
+          if constexpr (LM) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+          } else {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          }
         }
     }
     if (s + 1 < nst) sstore(buf ^ 1);
     __syncthreads();
   }
 
-  // epilogue: accumulator (i, j) element e of lane l is D[row][col] with
-  // row = m0 + wm*64 + 32 i + (l & 31), col = n0 + wn*64 + 32 j + 8 (e >> 2) + 4 (l >> 5) + (e & 3);
-  // output addresses as in the FP64 kernel (column-major with ldd, or the two-mode OutMap)
+  // epilogue: element e of lane l of accumulator (i, j) sits at lane index (l & 31) and register
+  // index 8 (e >> 2) + 4 (l >> 5) + (e & 3) of the 32x32 block; the lane index runs along M (rows
+  // of D) when LM, else along N.  Output addresses as in the FP64 kernel (column-major with ldd,
+  // or the two-mode OutMap); LM is chosen so that the lanes of a store follow the output's
+  // unit-stride axis.
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const int64_t row = m0 + wm * 64 + i * 32 + (lane & 31);
-    if (row >= M) continue;
-    const int64_t roff = om.rmod > 0 ? (row % om.rmod) * om.rs0 + (row / om.rmod) * om.rs1 : row;
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
-        const int64_t col = n0 + wn * 64 + j * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        if (col < N) {
-          int64_t coff;
-          if (om.rmod > 0) coff = om.cmod > 0 ? (col % om.cmod) * om.cs0 + (col / om.cmod) * om.cs1 : col * om.cs0;
-          else coff = col * ldd;
+        const int li = lane & 31, ri = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        const int64_t row = m0 + wm * 64 + i * 32 + (LM ? li : ri);
+        const int64_t col = n0 + wn * 64 + j * 32 + (LM ? ri : li);
+        if (row < M && col < N) {
+          int64_t off;
+          if (om.rmod > 0) {
+            const int64_t roff = (row % om.rmod) * om.rs0 + (row / om.rmod) * om.rs1;
+            const int64_t coff = om.cmod > 0 ? (col % om.cmod) * om.cs0 + (col / om.cmod) * om.cs1 : col * om.cs0;
+            off = roff + coff;
+          } else {
+            off = row + ldd * col;
+          }
           double val = alpha * (double)acc[i][j][e];
-          if (beta != 0.0) val += beta * (double)(om.rmod > 0 ? C[roff + coff] : C[row + ldc * col]);
-          D[roff + coff] = (TO)val;
+          if (beta != 0.0) val += beta * (double)(om.rmod > 0 ? C[off] : C[row + ldc * col]);
+          D[off] = (TO)val;
         }
       }
-  }
 }
 
 // L2 budget of the B blocks of one raster group (bytes)
@@ -280,12 +292,17 @@ static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double al
     gn = (int)std::max<int64_t>(1, std::min<int64_t>(tn, kGroupBytesBf3 / per));
   }
   const bool vec = aligned4(A, lda, batch > 1 ? sA : 0) && aligned4(B, ldb, batch > 1 ? sB : 0);
-  if (vec)
-    hipLaunchKernelGGL((k_gemm_bf3<TA, TB, TAG, TI, TO, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B,
-                       ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, gn, om);
-  else
-    hipLaunchKernelGGL((k_gemm_bf3<TA, TB, TAG, TI, TO, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B,
-                       ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, gn, om);
+  // lanes along M unless only the output's column axis has unit stride (mapped (T) outputs)
+  const bool lm = om.rmod <= 0 || om.rs0 == 1 || om.cs0 != 1;
+#define FH_BF3_GO(V_, L_)                                                                                            \
+  hipLaunchKernelGGL((k_gemm_bf3<TA, TB, TAG, TI, TO, V_, L_>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, \
+                     ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, gn, om)
+  if (TAG == 2 && !lm) {
+    if (vec) FH_BF3_GO(true, false); else FH_BF3_GO(false, false);
+  } else {
+    if (vec) FH_BF3_GO(true, true); else FH_BF3_GO(false, true);
+  }
+#undef FH_BF3_GO
   FH_HIP(hipGetLastError());
 }
 

@@ -328,10 +328,16 @@ void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, 
 // (the summand Ef of ijk.jl:132 is symmetric in (a,b,c); a=b=c terms vanish), times
 // (2 - d_ij - d_jk).  W[a,b,c] for the k-chunk lies in one buffer with a fastest
 // (W + a + v*b + v^2*c + v^3*kk).  The virtual space is cut into 8x8x8 cubes; one workgroup
-// takes an orbit {A,B,C} (A>=B>=C) of cube blocks, loads W on the (up to) six permuted cubes
-// into LDS (8-element runs along a), forms V = W + T1 x (ia|jb) terms and sums the orbit, so
-// every element of W is read once per triple.
+// takes an orbit {A,B,C} (A>=B>=C) of cube blocks and the (up to) six permuted cubes of it.
+// Each thread keeps the W values of its elements in registers and forms
+//     V = W + T1p[a,i] (jb|kc) + (ia|kc) T1p[b,j] + (ia|jb) T1p[c,k]          (ijk.jl:116)
+// from the orbit's T1 vectors and (ia|jb) blocks, staged once in LDS; the V cubes go to LDS
+// (rows padded to 9 and planes to 81 doubles, so the permuted reads of the energy sum spread
+// over the banks), and every element of W is read once per triple.
 static constexpr int CT = 8, CT3 = CT * CT * CT;
+static constexpr int VP = 648;  // padded cube: x + 9 y + 81 z
+
+__device__ __forceinline__ int vpad(int x, int y, int z) { return x + 9 * y + 81 * z; }
 
 template <typename TW>
 __global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int j, int k0, const TW* __restrict__ W,
@@ -339,48 +345,75 @@ __global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int
                                                         const double* __restrict__ t1p, const double* __restrict__ vvoo,
                                                         const double* __restrict__ eo, const double* __restrict__ ev,
                                                         double* __restrict__ partials) {
-  __shared__ double Wc[6][CT3];
-  __shared__ double Vc[6][CT3];
+  __shared__ double Vc[6][VP];
+  __shared__ double s_t1[3][3][CT];        // [occupied i/j/k][block A/B/C][8]
+  __shared__ double s_vv[3][3][3][CT * CT];  // [(jk) (ik) (ij)][block x][block y][x + 8 y]
+  __shared__ double s_ev[3][CT];
   __shared__ int slot_of[6];
   const int tid = threadIdx.x;
   const int kk = blockIdx.y, k = k0 + kk;
   const int ob = blockIdx.x;
   const double wocc = 2.0 - (i == j ? 1.0 : 0.0) - (j == k ? 1.0 : 0.0);
-  const int A = orbits[3 * ob], B = orbits[3 * ob + 1], C = orbits[3 * ob + 2];
-  // the six orderings of the orbit's blocks: (abc) (acb) (bac) (bca) (cab) (cba)
-  const int pb[6][3] = {{A, B, C}, {A, C, B}, {B, A, C}, {B, C, A}, {C, A, B}, {C, B, A}};
+  const int blk[3] = {orbits[3 * ob], orbits[3 * ob + 1], orbits[3 * ob + 2]};
+  // the six orderings of the orbit's blocks as indices into blk: (abc) (acb) (bac) (bca) (cab) (cba)
+  const int pb[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
   if (tid < 6) {
     int s = tid;
     for (int q = 0; q < tid; ++q)
-      if (pb[q][0] == pb[tid][0] && pb[q][1] == pb[tid][1] && pb[q][2] == pb[tid][2]) { s = q; break; }
+      if (blk[pb[q][0]] == blk[pb[tid][0]] && blk[pb[q][1]] == blk[pb[tid][1]] && blk[pb[q][2]] == blk[pb[tid][2]]) {
+        s = q;
+        break;
+      }
     slot_of[tid] = s;
   }
-  __syncthreads();
   double acc = 0.0;
   if (wocc != 0.0) {
     const int64_t vv = (int64_t)v * v;
+    const int occ[3] = {i, j, k};
+    // stage the orbit's T1 vectors, (ia|jb) blocks and virtual energies
+    for (int e = tid; e < 3 * 3 * CT; e += 256) {
+      const int x = e & 7, b = (e >> 3) % 3, q = e / 24;
+      const int a = blk[b] * CT + x;
+      s_t1[q][b][x] = a < v ? t1p[a + (int64_t)v * occ[q]] : 0.0;
+    }
+    for (int e = tid; e < 3 * 9 * CT * CT; e += 256) {
+      const int xy = e & 63, pr = (e >> 6) % 9, m = e / 576;
+      const int bx = pr % 3, by = pr / 3;
+      const int a = blk[bx] * CT + (xy & 7), b = blk[by] * CT + (xy >> 3);
+      const int oa = m == 0 ? j : i, ob2 = m == 2 ? j : k;   // (jk) (ik) (ij)
+      s_vv[m][bx][by][xy] = (a < v && b < v) ? vvoo[a + (int64_t)v * b + vv * (oa + (int64_t)o * ob2)] : 0.0;
+    }
+    if (tid < 3 * CT) {
+      const int a = blk[tid >> 3] * CT + (tid & 7);
+      s_ev[tid >> 3][tid & 7] = a < v ? ev[a] : 0.0;
+    }
+    __syncthreads();
     const TW* Wk = W + vv * v * kk;
-    const double* vjk = vvoo + vv * (j + (int64_t)o * k);
-    const double* vik = vvoo + vv * (i + (int64_t)o * k);
-    const double* vij = vvoo + vv * (i + (int64_t)o * j);
-    // load W and form V = W + T1p[x,i] vvoo[y,z,j,k] + vvoo[x,z,i,k] T1p[y,j] + vvoo[x,y,i,j] T1p[z,k]  (ijk.jl:116)
+    double wreg[6][2];
+#pragma unroll
     for (int p = 0; p < 6; ++p) {
+      wreg[p][0] = wreg[p][1] = 0.0;
       if (slot_of[p] != p) continue;
-      const int P0 = pb[p][0] * CT, Q0 = pb[p][1] * CT, R0 = pb[p][2] * CT;
-      for (int e = tid; e < CT3; e += 256) {
-        const int a = P0 + (e & 7), b = Q0 + ((e >> 3) & 7), c = R0 + (e >> 6);
+      const int pa = pb[p][0], pbb = pb[p][1], pc = pb[p][2];
+      const int P0 = blk[pa] * CT, Q0 = blk[pbb] * CT, R0 = blk[pc] * CT;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = tid + 256 * h;
+        const int la = e & 7, lb = (e >> 3) & 7, lc = e >> 6;
+        const int a = P0 + la, b = Q0 + lb, c = R0 + lc;
         double w = 0.0, val = 0.0;
         if (a < v && b < v && c < v) {
-          w = (double)Wk[a + (int64_t)v * b + vv * c];
-          val = w + t1p[a + (int64_t)v * i] * vjk[b + (int64_t)v * c] + vik[a + (int64_t)v * c] * t1p[b + (int64_t)v * j] +
-                vij[a + (int64_t)v * b] * t1p[c + (int64_t)v * k];
+          w = (double)Wk[a + v * b + vv * c];
+          val = w + s_t1[0][pa][la] * s_vv[0][pbb][pc][lb + 8 * lc] + s_vv[1][pa][pc][la + 8 * lc] * s_t1[1][pbb][lb] +
+                s_vv[2][pa][pbb][la + 8 * lb] * s_t1[2][pc][lc];
         }
-        Wc[p][e] = w;
-        Vc[p][e] = val;
+        wreg[p][h] = w;
+        Vc[p][vpad(la, lb, lc)] = val;
       }
     }
     __syncthreads();
     const double Dijk = eo[i] + eo[j] + eo[k];
+#pragma unroll
     for (int p = 0; p < 6; ++p) {
       if (slot_of[p] != p) continue;
       const int X = pb[p][0], Y = pb[p][1], Z = pb[p][2];
@@ -388,23 +421,23 @@ __global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int
       auto find = [&](int x0, int y0, int z0) {
         int r = 0;
         for (int q = 0; q < 6; ++q)
-          if (pb[q][0] == x0 && pb[q][1] == y0 && pb[q][2] == z0) { r = slot_of[q]; break; }
+          if (blk[pb[q][0]] == blk[x0] && blk[pb[q][1]] == blk[y0] && blk[pb[q][2]] == blk[z0]) { r = slot_of[q]; break; }
         return r;
       };
-      const int s_bca = find(Y, Z, X), s_cab = find(Z, X, Y), s_acb = find(X, Z, Y), s_bac = find(Y, X, Z),
-                s_cba = find(Z, Y, X);
-      for (int e = tid; e < CT3; e += 256) {
+      const double* Vbca = Vc[find(Y, Z, X)];
+      const double* Vcab = Vc[find(Z, X, Y)];
+      const double* Vacb = Vc[find(X, Z, Y)];
+      const double* Vbac = Vc[find(Y, X, Z)];
+      const double* Vcba = Vc[find(Z, Y, X)];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = tid + 256 * h;
         const int la = e & 7, lb = (e >> 3) & 7, lc = e >> 6;
-        const int a = X * CT + la, b = Y * CT + lb, c = Z * CT + lc;
-        if (a < v && b < v && c < v) {
-          const double Vabc = Vc[p][e];
-          const double Vbca = Vc[s_bca][lb + CT * lc + CT * CT * la];
-          const double Vcab = Vc[s_cab][lc + CT * la + CT * CT * lb];
-          const double Vacb = Vc[s_acb][la + CT * lc + CT * CT * lb];
-          const double Vbac = Vc[s_bac][lb + CT * la + CT * CT * lc];
-          const double Vcba = Vc[s_cba][lc + CT * lb + CT * CT * la];
-          const double Zs = 4.0 * Vabc + Vbca + Vcab - 2.0 * (Vacb + Vbac + Vcba);
-          acc += Wc[p][e] * Zs / (Dijk - ev[a] - ev[b] - ev[c]);
+        const double w = wreg[p][h];
+        if (w != 0.0) {
+          const double Zs = 4.0 * Vc[p][vpad(la, lb, lc)] + Vbca[vpad(lb, lc, la)] + Vcab[vpad(lc, la, lb)] -
+                            2.0 * (Vacb[vpad(la, lc, lb)] + Vbac[vpad(lb, la, lc)] + Vcba[vpad(lc, lb, la)]);
+          acc += w * Zs / (Dijk - s_ev[X][la] - s_ev[Y][lb] - s_ev[Z][lc]);
         }
       }
     }
