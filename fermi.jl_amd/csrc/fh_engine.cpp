@@ -674,9 +674,13 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   std::unique_ptr<DIIS> dm1(new DIIS(c, n1, op.cc_ndiis, f32)), dm2(new DIIS(c, n2, op.cc_ndiis, f32));
   c.iters.clear();
   // mixed precision: the residual's contractions run at `level` (2 split-BF16x3, 1 FP32,
-  // 0 FP64).  A reduced-precision phase ends when |dE| and rms fall below its switch thresholds;
-  // each switch starts a fresh DIIS subspace, and the FP64 phase polishes to the final criteria.
-  int level = op.precision_mode;
+  // 0 FP64).  A reduced-precision phase ends when |dE| and rms fall below its switch thresholds,
+  // or when its rms has not reached a new minimum for kStall iterations (the precision's own
+  // noise floor); the DIIS subspace is kept across switches, and the FP64 phase polishes to the
+  // final criteria (or for polish_iters iterations).
+  constexpr int kStall = 4;
+  int level = op.precision_mode, stall = 0;
+  double best_rms = 1e300;
   int polish_done = 0, niter_mixed = 0, niter_bf3 = 0;
   const double sw_e = std::max(op.mixed_e_switch, e_conv), sw_rms = std::max(op.mixed_rms_switch, max_rms);
   const double sw_e2 = std::max(op.bf3_e_switch, sw_e), sw_rms2 = std::max(op.bf3_rms_switch, sw_rms);
@@ -689,12 +693,13 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   int ite = 1;
   if (op.verbose) std::printf("%10s    %15s    %12s    %12s    %10s    %5s\n", "Iteration", "CC Energy", "dE", "Max RMS", "time (s)", "diis");
   while (true) {
-    while (level > 0 && std::fabs(dE) < (level == 2 ? sw_e2 : sw_e) && rms < (level == 2 ? sw_rms2 : sw_rms)) {
+    while (level > 0 && ((std::fabs(dE) < (level == 2 ? sw_e2 : sw_e) && rms < (level == 2 ? sw_rms2 : sw_rms)) ||
+                         stall >= kStall)) {
       --level;                                             // next precision level
-      dm1.reset(new DIIS(c, n1, op.cc_ndiis, f32));
-      dm2.reset(new DIIS(c, n2, op.cc_ndiis, f32));
       dE = 1.0;
       rms = 1.0;
+      stall = 0;
+      best_rms = 1e300;
     }
     if (level == 0) {
       if (!(std::fabs(dE) > e_conv || rms > max_rms)) break;                                   // :178
@@ -734,6 +739,12 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
     Ecc = energy(c, nT1.d(), nT2.d());                                                          // :218
     rms = std::max(r1, r2);
     dE = Ecc - oldE;
+    if (rms < best_rms) {
+      best_rms = rms;
+      stall = 0;
+    } else {
+      ++stall;
+    }
     if (level > 0) ++niter_mixed;
     if (level == 2) ++niter_bf3;
     if (level == 0 && op.precision_mode > 0) ++polish_done;
@@ -967,10 +978,10 @@ void fh_default_opts(fh_cc_opts* p) {
   p->verbose = 0;
   p->precision_mode = 0;
   p->polish_iters = -1;
-  p->mixed_e_switch = 1e-8;
-  p->mixed_rms_switch = 1e-7;
-  p->bf3_e_switch = 1e-6;
-  p->bf3_rms_switch = 1e-5;
+  p->mixed_e_switch = 1e-9;
+  p->mixed_rms_switch = 1e-9;
+  p->bf3_e_switch = 1e-5;
+  p->bf3_rms_switch = 1e-6;
 }
 
 const char* fh_last_error(void) { return g_last_error.c_str(); }

@@ -33,8 +33,7 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 namespace {
 
-constexpr int kBM = 128, kBN = 128, kBK = 32;
-constexpr int kRow = kBK + 8;  // LDS row of one operand tile: 32 bf16 + 16 bytes of padding
+constexpr int kBM = 128, kBN = 128;
 
 // 4 values of an operand, as FP32
 struct V4 {
@@ -54,6 +53,14 @@ __device__ inline void split2(float a, float b, uint32_t& hi, uint32_t& lo) {
   __builtin_memcpy(&lb, &l, 4);
   hi = hb;
   lo = lb;
+}
+
+// split 2 values and write them to LDS rows (4 bytes per plane)
+__device__ inline void put2(uint16_t* hi_plane, uint16_t* lo_plane, int off, float a, float b) {
+  uint32_t h, l;
+  split2(a, b, h, l);
+  *reinterpret_cast<uint32_t*>(hi_plane + off) = h;
+  *reinterpret_cast<uint32_t*>(lo_plane + off) = l;
 }
 
 // split 4 values and write them to LDS rows (8 bytes per plane)
@@ -89,15 +96,21 @@ __device__ inline V4 load4(const TI* p, bool full, int n_ok) {
 
 }  // namespace
 
-// Operand tile layout in LDS: plane[row][k] (k fastest, kRow bf16 per row), row = m for A, n for B.
-// "row-contiguous" operands (A with TA = false, B with TB = true) are read 4 rows x 4 k per thread
-// and transposed in registers; "k-contiguous" operands are read 4 k of one row per load.
-template <bool TA, bool TB, int TAG, typename TI, typename TO, bool VEC, bool LM>
-__global__ __launch_bounds__(256, 2) void k_gemm_bf3(int64_t M, int64_t N, int64_t K, double alpha,
+// Operand tile layout in LDS: plane[row][k] (k fastest, BK + 8 bf16 per row: 16 bytes of padding),
+// row = m for A, n for B.  "row-contiguous" operands (A with TA = false, B with TB = true) are
+// read 4 rows x BK/8 k per thread and transposed in registers; "k-contiguous" operands are read
+// 4 k of one row per load.  BK = 32 (two workgroups per CU) for long K, BK = 16 (three per CU,
+// more overlap of one tile's epilogue with another's loop) for short K.
+template <int BK, bool TA, bool TB, int TAG, typename TI, typename TO, bool VEC, bool LM>
+__global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, int64_t N, int64_t K, double alpha,
                                                      const TI* __restrict__ A, int64_t lda, int64_t sA,
                                                      const TI* __restrict__ B, int64_t ldb, int64_t sB, double beta,
                                                      const TO* C, int64_t ldc, int64_t sC, TO* D, int64_t ldd,
                                                      int64_t sD, int tilesM, int tilesN, int gn, OutMap om) {
+  constexpr int kRow = BK + 8;   // LDS row (bf16)
+  constexpr int KG = BK / 8;      // k per thread (row-contiguous) = loads per thread (both patterns)
+  constexpr int KQ = BK / 4;      // 4-wide k groups (k-contiguous)
+  constexpr int RS = 256 / KQ;    // row step (k-contiguous)
   __shared__ __attribute__((aligned(16))) uint16_t As[2][2][kBM * kRow];  // [buf][hi/lo]
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][2][kBN * kRow];
 
@@ -127,24 +140,24 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf3(int64_t M, int64_t N, int64
   D += bz * sD;
 
   // per-thread load coordinates
-  // row-contiguous pattern: rows 4*(tid&31)..+3, k 4*(tid>>5)..+3 (one load per k)
-  // k-contiguous pattern:  k 4*(tid&7)..+3, rows (tid>>3) + 32 j, j = 0..3
-  V4 ra[4], rb[4];
+  // row-contiguous pattern: rows 4*(tid&31)..+3, k KG*(tid>>5)..+KG-1 (one load per k)
+  // k-contiguous pattern:  k 4*(tid%KQ)..+3, rows tid/KQ + RS j, j < KG
+  V4 ra[KG], rb[KG];
   auto gload = [&](int64_t k0) {
     if (!TA) {
       const int64_t gm = m0 + 4 * (tid & 31);
       const int n_ok = clamp4(M - gm);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t gk = k0 + 4 * (tid >> 5) + j;
+      for (int j = 0; j < KG; ++j) {
+        const int64_t gk = k0 + KG * (tid >> 5) + j;
         ra[j] = load4<TI, VEC>(A + gm + lda * gk, n_ok == 4 && gk < K, gk < K ? n_ok : 0);
       }
     } else {
-      const int64_t gk = k0 + 4 * (tid & 7);
+      const int64_t gk = k0 + 4 * (tid % KQ);
       const int k_ok = clamp4(K - gk);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t gm = m0 + (tid >> 3) + 32 * j;
+      for (int j = 0; j < KG; ++j) {
+        const int64_t gm = m0 + tid / KQ + RS * j;
         ra[j] = load4<TI, VEC>(A + gk + lda * gm, k_ok == 4 && gm < M, gm < M ? k_ok : 0);
       }
     }
@@ -152,42 +165,42 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf3(int64_t M, int64_t N, int64
       const int64_t gn_ = n0 + 4 * (tid & 31);
       const int n_ok = clamp4(N - gn_);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t gk = k0 + 4 * (tid >> 5) + j;
+      for (int j = 0; j < KG; ++j) {
+        const int64_t gk = k0 + KG * (tid >> 5) + j;
         rb[j] = load4<TI, VEC>(B + gn_ + ldb * gk, n_ok == 4 && gk < K, gk < K ? n_ok : 0);
       }
     } else {
-      const int64_t gk = k0 + 4 * (tid & 7);
+      const int64_t gk = k0 + 4 * (tid % KQ);
       const int k_ok = clamp4(K - gk);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t gcol = n0 + (tid >> 3) + 32 * j;
+      for (int j = 0; j < KG; ++j) {
+        const int64_t gcol = n0 + tid / KQ + RS * j;
         rb[j] = load4<TI, VEC>(B + gk + ldb * gcol, k_ok == 4 && gcol < N, gcol < N ? k_ok : 0);
       }
     }
   };
+  // r[j].x[i] = X[row 4(tid&31)+i][k KG(tid>>5)+j]: gather the KG k of each row
+  auto put_rows = [&](uint16_t* hi, uint16_t* lo, const V4* r) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int off = (4 * (tid & 31) + i) * kRow + KG * (tid >> 5);
+      if constexpr (KG == 4) {
+        const V4 t = {{r[0].x[i], r[1].x[i], r[2].x[i], r[3].x[i]}};
+        put4(hi, lo, off, t);
+      } else {
+        put2(hi, lo, off, r[0].x[i], r[1].x[i]);
+      }
+    }
+  };
+  auto put_k = [&](uint16_t* hi, uint16_t* lo, const V4* r) {
+#pragma unroll
+    for (int j = 0; j < KG; ++j) put4(hi, lo, (tid / KQ + RS * j) * kRow + 4 * (tid % KQ), r[j]);
+  };
   auto sstore = [&](int buf) {
-    if (!TA) {
-      // ra[j].x[i] = A[row 4(tid&31)+i][k 4(tid>>5)+j]: gather the 4 k of each row
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const V4 t = {{ra[0].x[i], ra[1].x[i], ra[2].x[i], ra[3].x[i]}};
-        put4(As[buf][0], As[buf][1], (4 * (tid & 31) + i) * kRow + 4 * (tid >> 5), t);
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) put4(As[buf][0], As[buf][1], ((tid >> 3) + 32 * j) * kRow + 4 * (tid & 7), ra[j]);
-    }
-    if (TB) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const V4 t = {{rb[0].x[i], rb[1].x[i], rb[2].x[i], rb[3].x[i]}};
-        put4(Bs[buf][0], Bs[buf][1], (4 * (tid & 31) + i) * kRow + 4 * (tid >> 5), t);
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) put4(Bs[buf][0], Bs[buf][1], ((tid >> 3) + 32 * j) * kRow + 4 * (tid & 7), rb[j]);
-    }
+    if (!TA) put_rows(As[buf][0], As[buf][1], ra);
+    else put_k(As[buf][0], As[buf][1], ra);
+    if (TB) put_rows(Bs[buf][0], Bs[buf][1], rb);
+    else put_k(Bs[buf][0], Bs[buf][1], rb);
   };
 
   f32x16 acc[2][2];
@@ -198,16 +211,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf3(int64_t M, int64_t N, int64
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
 
-  const int64_t nst = (K + kBK - 1) / kBK;
+  const int64_t nst = (K + BK - 1) / BK;
   gload(0);
   sstore(0);
   __syncthreads();
   const int frow = lane & 31, fk = 8 * (lane >> 5);
   for (int64_t s = 0; s < nst; ++s) {
     const int buf = s & 1;
-    if (s + 1 < nst) gload((s + 1) * kBK);
+    if (s + 1 < nst) gload((s + 1) * BK);
 #pragma unroll
-    for (int kk = 0; kk < kBK / 16; ++kk) {
+    for (int kk = 0; kk < BK / 16; ++kk) {
       bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
@@ -294,13 +307,20 @@ static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double al
   const bool vec = aligned4(A, lda, batch > 1 ? sA : 0) && aligned4(B, ldb, batch > 1 ? sB : 0);
   // lanes along M unless only the output's column axis has unit stride (mapped (T) outputs)
   const bool lm = om.rmod <= 0 || om.rs0 == 1 || om.cs0 != 1;
-#define FH_BF3_GO(V_, L_)                                                                                            \
-  hipLaunchKernelGGL((k_gemm_bf3<TA, TB, TAG, TI, TO, V_, L_>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, \
-                     ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, gn, om)
-  if (TAG == 2 && !lm) {
-    if (vec) FH_BF3_GO(true, false); else FH_BF3_GO(false, false);
+#define FH_BF3_GO(BK_, V_, L_)                                                                                        \
+  hipLaunchKernelGGL((k_gemm_bf3<BK_, TA, TB, TAG, TI, TO, V_, L_>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, \
+                     B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, gn, om)
+  if (TAG == 2) {
+    // the (T) W build: K = v + o is short
+    if (lm) {
+      if (vec) FH_BF3_GO(16, true, true); else FH_BF3_GO(16, false, true);
+    } else {
+      if (vec) FH_BF3_GO(16, true, false); else FH_BF3_GO(16, false, false);
+    }
+  } else if (K <= 512) {
+    if (vec) FH_BF3_GO(16, true, true); else FH_BF3_GO(16, false, true);
   } else {
-    if (vec) FH_BF3_GO(true, true); else FH_BF3_GO(false, true);
+    if (vec) FH_BF3_GO(32, true, true); else FH_BF3_GO(32, false, true);
   }
 #undef FH_BF3_GO
   FH_HIP(hipGetLastError());

@@ -34,17 +34,16 @@ from fermihip import synth  # noqa: E402
 DEFAULT_MODEL = os.path.join(HERE, "t2_guess_water.npz")
 
 
-def cluster_features(T2mp2, eo, ev, dij):
-    """Raw feature rows [n, 6] of every amplitude, column-major element order (as on the GPU)."""
+def cluster_features(T2mp2, eo, ev, dij, elems=None):
+    """Raw feature rows [n, 6] of the amplitudes with column-major element indices `elems`
+    (all of them when None), in the order the GPU feature kernel uses."""
     o, _, v, _ = T2mp2.shape
-    cols = [T2mp2.ravel(order="F")]
-    e = np.arange(o * o * v * v)
+    e = np.arange(o * o * v * v) if elems is None else np.asarray(elems)
     i = e % o
     j = (e // o) % o
     a = (e // (o * o)) % v
     b = e // (o * o * v)
-    cols += [eo[i], eo[j], ev[a], ev[b], dij[i, j]]
-    return np.stack(cols, axis=1)
+    return np.stack([T2mp2.ravel(order="F")[e], eo[i], eo[j], ev[a], ev[b], dij[i, j]], axis=1)
 
 
 def converged_cluster(nw, seed, opts):
@@ -106,15 +105,15 @@ def main():
             if not r.converged:
                 print(f"train_guess: skip (H2O){nw} seed {seed}: not converged", flush=True)
                 continue
-            F = cluster_features(T2m, eo, ev, dij)
-            dT = (T2c - T2m).ravel(order="F")
-            n = F.shape[0]
+            tm = T2m.ravel(order="F")
+            n = tm.size
             m = min(n, args.samples)
-            w = np.abs(F[:, 0]) + 1e-30
-            pick = np.concatenate([rng.choice(n, m // 2, replace=False),
-                                   rng.choice(n, m - m // 2, replace=False, p=w / w.sum())])
-            Xs.append(F[pick])
-            Ys.append(dT[pick])
+            w = np.abs(tm) + 1e-30
+            # half uniform, half |T2_MP2|-weighted (drawn with replacement, duplicates dropped)
+            pick = np.unique(np.concatenate([rng.choice(n, m // 2, replace=False),
+                                             rng.choice(n, m - m // 2, replace=True, p=w / w.sum())]))
+            Xs.append(cluster_features(T2m, eo, ev, dij, pick))
+            Ys.append((T2c.ravel(order="F") - tm)[pick])
             nsys += 1
         print(f"train_guess: (H2O){nw} done, {nsys} clusters, {time.perf_counter() - t0:.1f} s", flush=True)
     X = np.concatenate(Xs)
