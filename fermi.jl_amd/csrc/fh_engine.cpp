@@ -305,13 +305,18 @@ static void contract(Ctx& c, double alpha, const TView& A, const std::string& ia
   const int64_t wsz = gemm_splitk_elems(Msz, Nsz, Ksz, 1);
   Buf work;
   if (wsz) work = dalloc(c, wsz);
-  // split-BF16x3 for the o^3 v^3-class GEMMs of the mixed phase; smaller ones (and the split-K
-  // shapes) stay on FP32 MFMA
+  // split-BF16x3 for the o^3 v^3-class GEMMs of the mixed phase, on FP32 copies of the operands
+  // (half the operand traffic of FP64 loads); smaller GEMMs (and the split-K shapes) stay on FP32 MFMA
   const bool bf3 = c.lowp == 2 && wsz == 0 && (double)Msz * (double)Nsz * (double)Ksz >= 1e9;
   auto gemm = [&](bool ta, bool tb, int64_t m, int64_t n, double al, const double* a, int64_t la, const double* b,
                   int64_t lb, double be, const double* cc, double* d, int64_t ld) {
-    if (bf3) gemm_bf3<double, double>(c.s, ta, tb, m, n, Ksz, al, a, la, 0, b, lb, 0, be, cc, ld, 0, d, ld, 0, 1, 0, nullptr);
-    else gemm_f64(c.s, ta, tb, m, n, Ksz, al, a, la, 0, b, lb, 0, be, cc, ld, 0, d, ld, 0, 1, 0, work.d(), wsz, nullptr,
+    if (bf3) {
+      Buf af(c.pool.get(), (size_t)(m * Ksz) * sizeof(float)), bf(c.pool.get(), (size_t)(n * Ksz) * sizeof(float));
+      cast_f64_to_f32(c.s, m * Ksz, a, af.f());
+      cast_f64_to_f32(c.s, n * Ksz, b, bf.f());
+      gemm_bf3<float, double>(c.s, ta, tb, m, n, Ksz, al, af.f(), la, 0, bf.f(), lb, 0, be, cc, ld, 0, d, ld, 0, 1, 0,
+                              nullptr);
+    } else gemm_f64(c.s, ta, tb, m, n, Ksz, al, a, la, 0, b, lb, 0, be, cc, ld, 0, d, ld, 0, 1, 0, work.d(), wsz, nullptr,
                   c.lowp != 0);
   };
   if (cmode == 0) {
@@ -676,8 +681,9 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   // mixed precision: the residual's contractions run at `level` (2 split-BF16x3, 1 FP32,
   // 0 FP64).  A reduced-precision phase ends when |dE| and rms fall below its switch thresholds,
   // or when its rms has not reached a new minimum for kStall iterations (the precision's own
-  // noise floor); the DIIS subspace is kept across switches, and the FP64 phase polishes to the
-  // final criteria (or for polish_iters iterations).
+  // noise floor).  Each switch starts a fresh DIIS subspace: the error vectors of the lower
+  // precision describe its own fixed point, which extrapolation would otherwise pull back to.
+  // The FP64 phase polishes to the final criteria (or for polish_iters iterations).
   constexpr int kStall = 4;
   int level = op.precision_mode, stall = 0;
   double best_rms = 1e300;
@@ -696,6 +702,8 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
     while (level > 0 && ((std::fabs(dE) < (level == 2 ? sw_e2 : sw_e) && rms < (level == 2 ? sw_rms2 : sw_rms)) ||
                          stall >= kStall)) {
       --level;                                             // next precision level
+      dm1.reset(new DIIS(c, n1, op.cc_ndiis, f32));
+      dm2.reset(new DIIS(c, n2, op.cc_ndiis, f32));
       dE = 1.0;
       rms = 1.0;
       stall = 0;
@@ -770,19 +778,20 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
 
 // ------------------------------------------------------------------ (T), ijk.jl:20-150
 template <typename TO>
-__global__ void k_build_av(int o, int v, bool swap_xy, const double* __restrict__ ovvv, const double* __restrict__ t2,
-                           TO* __restrict__ av) {
+__global__ void k_build_av(int o, int v, int ndp, bool swap_xy, const double* __restrict__ ovvv,
+                           const double* __restrict__ t2, TO* __restrict__ av) {
   // AV[(x,y), n, d'] : d' < v  -> (n y | x d') = OVVV[n, y, x, d']   (ijk2.jl:33 vvvo layout)
-  //                    d' >= v -> -T2[n, l, y, x], l = d' - v       (T2_n with the GEMM's minus sign)
+  //                    v <= d' < v+o -> -T2[n, l, y, x], l = d' - v  (T2_n with the GEMM's minus sign)
+  //                    d' >= v+o -> 0 (K padded to ndp, a multiple of 4, for aligned vector loads)
   // swap_xy stores the rows as (y,x) instead
   const int64_t vv = (int64_t)v * v, nd = (int64_t)v + o;
-  const int64_t total = vv * o * nd;
+  const int64_t total = vv * o * ndp;
   for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
     const int64_t r0 = idx % v, r1 = (idx / v) % v, n = (idx / vv) % o, dp = idx / (vv * o);
     const int64_t x = swap_xy ? r1 : r0, y = swap_xy ? r0 : r1;
-    double val;
+    double val = 0.0;
     if (dp < v) val = ovvv[n + (int64_t)o * (y + (int64_t)v * (x + (int64_t)v * dp))];
-    else {
+    else if (dp < nd) {
       const int64_t l = dp - v;
       val = -t2[n + (int64_t)o * (l + (int64_t)o * (y + (int64_t)v * x))];
     }
@@ -790,18 +799,20 @@ __global__ void k_build_av(int o, int v, bool swap_xy, const double* __restrict_
   }
 }
 template <typename TO>
-__global__ void k_build_bv(int o, int v, bool swap_pq, const double* __restrict__ t2, const double* __restrict__ ooov,
-                           TO* __restrict__ bv) {
-  // Bcat_{p,q}[d', z]: d' < v -> T2p[d', z, p, q] = T2[q, p, z, d'];  d' >= v -> ovoo[l, z, p, q] = OOOV[l, p, q, z]
-  // stored as bv[d', z, p, q] (swap_pq = false) or bv[d', z, q, p] (swap_pq = true)
+__global__ void k_build_bv(int o, int v, int ndp, bool swap_pq, const double* __restrict__ t2,
+                           const double* __restrict__ ooov, TO* __restrict__ bv) {
+  // Bcat_{p,q}[d', z]: d' < v -> T2p[d', z, p, q] = T2[q, p, z, d'];  v <= d' < v+o -> ovoo[l, z, p, q] =
+  // OOOV[l, p, q, z]; d' >= v+o -> 0 (padding); stored as bv[d', z, p, q] (swap_pq = false) or
+  // bv[d', z, q, p] (swap_pq = true) with leading dimension ndp
   const int64_t nd = (int64_t)v + o;
-  const int64_t total = nd * v * o * o;
+  const int64_t total = (int64_t)ndp * v * o * o;
   for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
-    const int64_t dp = idx % nd, z = (idx / nd) % v, r = (idx / (nd * v)) % o, s = idx / (nd * v * o);
+    const int64_t dp = idx % ndp, z = (idx / ndp) % v, r = (idx / ((int64_t)ndp * v)) % o,
+                  s = idx / ((int64_t)ndp * v * o);
     const int64_t p = swap_pq ? s : r, q = swap_pq ? r : s;
-    double val;
+    double val = 0.0;
     if (dp < v) val = t2[q + (int64_t)o * (p + (int64_t)o * (z + (int64_t)v * dp))];
-    else {
+    else if (dp < nd) {
       const int64_t l = dp - v;
       val = ooov[l + (int64_t)o * (p + (int64_t)o * (q + (int64_t)o * z))];
     }
@@ -825,19 +836,19 @@ __global__ void k_accumulate(int64_t n, const double* __restrict__ partials, dou
 // one W-build GEMM of the (T) driver (tag 2), FP64 or FP32 data
 static void tgemm(Ctx& c, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, int64_t sA, const double* B,
                   int64_t ldb, int64_t sB, double beta, double* W, int64_t ldd, int64_t sD, int64_t batch,
-                  const OutMap* map, bool) {
+                  const OutMap* map, bool, const KSeg* seg) {
   gemm_f64(c.s, false, false, M, N, K, 1.0, A, lda, sA, B, ldb, sB, beta, beta != 0.0 ? W : nullptr, ldd, sD, W, ldd, sD,
-           batch, 2, nullptr, 0, map, false);
+           batch, 2, nullptr, 0, map, false, seg);
 }
 static void tgemm(Ctx& c, int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, int64_t sA, const float* B,
                   int64_t ldb, int64_t sB, double beta, float* W, int64_t ldd, int64_t sD, int64_t batch,
-                  const OutMap* map, bool bf3) {
+                  const OutMap* map, bool bf3, const KSeg* seg) {
   if (bf3)
     gemm_bf3<float, float>(c.s, false, false, M, N, K, 1.0, A, lda, sA, B, ldb, sB, beta, beta != 0.0 ? W : nullptr, ldd,
-                           sD, W, ldd, sD, batch, 2, map);
+                           sD, W, ldd, sD, batch, 2, map, seg);
   else
     gemm_f32_ff(c.s, M, N, K, 1.0, A, lda, sA, B, ldb, sB, beta, beta != 0.0 ? W : nullptr, ldd, sD, W, ldd, sD, batch,
-                2, map);
+                2, map, seg);
 }
 
 // (T) with W-build operands and W stored as T (double: FP64 path; float: FP32 MFMA path, or
@@ -847,7 +858,8 @@ static double triples_t(Ctx& c, bool bf3) {
   prepare(c);
   FH_CHECK(c.have_amp, -6, "no amplitudes for (T)");
   const bool lowp = std::is_same<T, float>::value;
-  const int64_t o = c.o, v = c.v, vv = v * v, nd = v + o;
+  const int64_t o = c.o, v = c.v, vv = v * v;
+  const int64_t nd = ((v + o + 3) / 4) * 4;  // K = v + o padded with zeros to a multiple of 4
   const double o_ = (double)o, v_ = (double)v;
   const double ntrip = o_ * (o_ + 1) * (o_ + 2) / 6.0;
   const std::string sfx = bf3 ? "_bf16x3" : (lowp ? "_fp32" : "");
@@ -863,10 +875,14 @@ static double triples_t(Ctx& c, bool bf3) {
   Buf T1p = dalloc(c, v * o), vvoo = dalloc(c, vv * o * o);
   {
     auto g = [](int64_t n) { int64_t b = (n + 255) / 256; return (unsigned)std::min<int64_t>(b, 65536 * 4); };
-    hipLaunchKernelGGL(k_build_av<T>, dim3(g(avsz)), dim3(256), 0, c.s, (int)o, (int)v, false, c.OVVV.d(), c.T2.d(), AV);
-    hipLaunchKernelGGL(k_build_av<T>, dim3(g(avsz)), dim3(256), 0, c.s, (int)o, (int)v, true, c.OVVV.d(), c.T2.d(), AVs);
-    hipLaunchKernelGGL(k_build_bv<T>, dim3(g(bvsz)), dim3(256), 0, c.s, (int)o, (int)v, false, c.T2.d(), c.OOOV.d(), BVp);
-    hipLaunchKernelGGL(k_build_bv<T>, dim3(g(bvsz)), dim3(256), 0, c.s, (int)o, (int)v, true, c.T2.d(), c.OOOV.d(), BVq);
+    hipLaunchKernelGGL(k_build_av<T>, dim3(g(avsz)), dim3(256), 0, c.s, (int)o, (int)v, (int)nd, false, c.OVVV.d(),
+                       c.T2.d(), AV);
+    hipLaunchKernelGGL(k_build_av<T>, dim3(g(avsz)), dim3(256), 0, c.s, (int)o, (int)v, (int)nd, true, c.OVVV.d(),
+                       c.T2.d(), AVs);
+    hipLaunchKernelGGL(k_build_bv<T>, dim3(g(bvsz)), dim3(256), 0, c.s, (int)o, (int)v, (int)nd, false, c.T2.d(),
+                       c.OOOV.d(), BVp);
+    hipLaunchKernelGGL(k_build_bv<T>, dim3(g(bvsz)), dim3(256), 0, c.s, (int)o, (int)v, (int)nd, true, c.T2.d(),
+                       c.OOOV.d(), BVq);
     FH_HIP(hipGetLastError());
     perm(c, tv(c.T1.d(), {o, v}), "ia", tv(T1p.d(), {v, o}), "ai", 1.0, 0.0);
     perm(c, tv(c.OVOV.d(), {o, v, o, v}), "iajb", tv(vvoo.d(), {v, v, o, o}), "abij", 1.0, 0.0);
@@ -887,9 +903,7 @@ static double triples_t(Ctx& c, bool bf3) {
   fill_f64(c.s, 1, 0.0, acc.d());
   const int64_t bstride = nd * v;  // one (p,q) column block of BVp/BVq
   // output maps into W[a + v*b + v^2*c + v^3*kk]
-  OutMap m_cab, m_cba, m_acb, m_bca;
-  m_cab.rmod = v; m_cab.rs0 = 1; m_cab.rs1 = vv; m_cab.cmod = v; m_cab.cs0 = v; m_cab.cs1 = v3;   // rows (a,c), cols (b,kk)
-  m_cba.rmod = v; m_cba.rs0 = v; m_cba.rs1 = vv; m_cba.cmod = v; m_cba.cs0 = 1; m_cba.cs1 = v3;   // rows (b,c), cols (a,kk)
+  OutMap m_acb, m_bca;
   m_acb.rmod = v; m_acb.rs0 = 1; m_acb.rs1 = vv; m_acb.cmod = 0; m_acb.cs0 = v;                   // rows (a,c), col b
   m_bca.rmod = v; m_bca.rs0 = v; m_bca.rs1 = vv; m_bca.cmod = 0; m_bca.cs0 = 1;                   // rows (b,c), col a
   for (int64_t i = 0; i < o; ++i)
@@ -898,21 +912,25 @@ static double triples_t(Ctx& c, bool bf3) {
       for (int64_t k0 = 0; k0 < kend; k0 += KC) {
         const int64_t nk = std::min<int64_t>(KC, kend - k0);
         std::unique_ptr<Timer> tg(new Timer(c, "triples_gemm" + sfx,
-                                            12.0 * (double)vv * (double)v * (double)nd * (double)nk));
+                                            12.0 * (double)vv * (double)v * (double)(v + o) * (double)nk));
         // The six permutation classes of ijk2.jl:109-153, G = AV_n (rows (x,y)) * Bcat_{p,q}
-        // (cols z), accumulated into W[a,b,c,kk] (W[a,b,c] += G[class-permuted (a,b,c)]).
-        // abc: n=j, B_{i,k}, x=a y=b z=c
-        tgemm(c, vv, v * nk, nd, AV + vv * j, vv * o, 0, BVq + bstride * (k0 + o * i), nd, 0, 0.0, W, vv, 0, 1, nullptr, bf3);
-        // bac: n=i, B_{j,k}, x=b y=a z=c  (row-swapped AV: rows (a,b))
-        tgemm(c, vv, v * nk, nd, AVs + vv * i, vv * o, 0, BVq + bstride * (k0 + o * j), nd, 0, 1.0, W, vv, 0, 1, nullptr, bf3);
-        // cab: n=i, B_{k,j}, x=c y=a z=b  (rows (a,c), cols (b,kk))
-        tgemm(c, vv, v * nk, nd, AVs + vv * i, vv * o, 0, BVp + bstride * (k0 + o * j), nd, 0, 1.0, W, vv, 0, 1, &m_cab, bf3);
-        // cba: n=j, B_{k,i}, x=c y=b z=a  (rows (b,c), cols (a,kk))
-        tgemm(c, vv, v * nk, nd, AVs + vv * j, vv * o, 0, BVp + bstride * (k0 + o * i), nd, 0, 1.0, W, vv, 0, 1, &m_cba, bf3);
-        // acb: n=k, B_{i,j}, x=a y=c z=b  (batched over k: rows (a,c), col b)
-        tgemm(c, vv, v, nd, AV + vv * k0, vv * o, vv, BVp + bstride * (i + o * j), nd, 0, 1.0, W, vv, v3, nk, &m_acb, bf3);
-        // bca: n=k, B_{j,i}, x=b y=c z=a  (batched over k: rows (b,c), col a)
-        tgemm(c, vv, v, nd, AV + vv * k0, vv * o, vv, BVp + bstride * (j + o * i), nd, 0, 1.0, W, vv, v3, nk, &m_bca, bf3);
+        // (cols z), accumulated into W[a,b,c,kk] (W[a,b,c] += G[class-permuted (a,b,c)]).  The two
+        // classes of each output layout form one GEMM with two K segments:
+        //   abc (n=j, B_{i,k}, rows (a,b), cols (c,kk)) + bac (n=i row-swapped, B_{j,k}): identity map;
+        //   cab (n=i row-swapped, B_{k,j}) + acb (n=k, B_{i,j}): rows (a,c), col b, batched over kk;
+        //   cba (n=j row-swapped, B_{k,i}) + bca (n=k, B_{j,i}): rows (b,c), col a, batched over kk.
+        KSeg s_bac, s_acb, s_bca;
+        s_bac.A = AVs + vv * i; s_bac.lda = vv * o; s_bac.B = BVq + bstride * (k0 + o * j); s_bac.ldb = nd; s_bac.K1 = nd;
+        tgemm(c, vv, v * nk, 2 * nd, AV + vv * j, vv * o, 0, BVq + bstride * (k0 + o * i), nd, 0, 0.0, W, vv, 0, 1,
+              nullptr, bf3, &s_bac);
+        s_acb.A = AV + vv * k0; s_acb.lda = vv * o; s_acb.sA = vv; s_acb.B = BVp + bstride * (i + o * j); s_acb.ldb = nd;
+        s_acb.K1 = nd;
+        tgemm(c, vv, v, 2 * nd, AVs + vv * i, vv * o, 0, BVp + bstride * (k0 + o * j), nd, bstride, 1.0, W, vv, v3, nk,
+              &m_acb, bf3, &s_acb);
+        s_bca.A = AV + vv * k0; s_bca.lda = vv * o; s_bca.sA = vv; s_bca.B = BVp + bstride * (j + o * i); s_bca.ldb = nd;
+        s_bca.K1 = nd;
+        tgemm(c, vv, v, 2 * nd, AVs + vv * j, vv * o, 0, BVp + bstride * (k0 + o * i), nd, bstride, 1.0, W, vv, v3, nk,
+              &m_bca, bf3, &s_bca);
         tg.reset();
         Timer te(c, "triples_energy" + sfx, 0.0);
         int64_t nb = 0;

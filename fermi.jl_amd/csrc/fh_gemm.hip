@@ -28,11 +28,12 @@ namespace fh {
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-template <int BM, int BN, int BK, bool TA, bool TB, int OCC, int TAG, bool F32, typename TI, typename TO, bool LM>
+template <int BM, int BN, int BK, bool TA, bool TB, int OCC, int TAG, bool F32, typename TI, typename TO, bool LM,
+          bool SEG>
 __global__ __launch_bounds__(256, OCC) void k_gemm(
     int64_t M, int64_t N, int64_t K, double alpha, const TI* __restrict__ A, int64_t lda, int64_t sA,
     const TI* __restrict__ B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc, int64_t sC, TO* D,
-    int64_t ldd, int64_t sD, int tilesM, int tilesN, int64_t kchunk, int gn, OutMap om) {
+    int64_t ldd, int64_t sD, int tilesM, int tilesN, int64_t kchunk, int gn, OutMap om, KSeg sg) {
   static_assert(F32 || std::is_same<TI, double>::value, "FP64 MFMA needs FP64 operands");
   using T = typename std::conditional<F32, float, double>::type;
   using Acc = typename std::conditional<F32, f32x4, f64x4>::type;
@@ -76,6 +77,10 @@ __global__ __launch_bounds__(256, OCC) void k_gemm(
   }
   if (C) C += bz * sC;
   D += bz * sD;
+  // second K segment (SEG; split-K is never combined with it)
+  const TI* A2 = SEG ? static_cast<const TI*>(sg.A) + bz * sg.sA : nullptr;
+  const TI* B2 = SEG ? static_cast<const TI*>(sg.B) + bz * sg.sB : nullptr;
+  const int64_t K1 = SEG ? sg.K1 : K;
 
   TI ra[LA], rb[LB];
   auto gload = [&](int64_t k0) {
@@ -86,7 +91,10 @@ __global__ __launch_bounds__(256, OCC) void k_gemm(
       if (!TA) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
       const int64_t gm = m0 + mm, gk = k0 + kk;
       TI val = 0;
-      if (gm < M && gk < K) val = TA ? A[gk + lda * gm] : A[gm + lda * gk];
+      if (gm < M && gk < K) {
+        if (!SEG || gk < K1) val = TA ? A[gk + lda * gm] : A[gm + lda * gk];
+        else val = TA ? A2[(gk - K1) + sg.lda * gm] : A2[gm + sg.lda * (gk - K1)];
+      }
       ra[t] = val;
     }
 #pragma unroll
@@ -96,7 +104,10 @@ __global__ __launch_bounds__(256, OCC) void k_gemm(
       if (!TB) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
       const int64_t gcol = n0 + nn, gk = k0 + kk;
       TI val = 0;
-      if (gcol < N && gk < K) val = TB ? B[gcol + ldb * gk] : B[gk + ldb * gcol];
+      if (gcol < N && gk < K) {
+        if (!SEG || gk < K1) val = TB ? B[gcol + ldb * gk] : B[gk + ldb * gcol];
+        else val = TB ? B2[gcol + sg.ldb * (gk - K1)] : B2[(gk - K1) + sg.ldb * gcol];
+      }
       rb[t] = val;
     }
   };
@@ -200,7 +211,7 @@ static constexpr int64_t kGroupBytes = 1 << 20;
 template <int BM, int BN, int BK, bool TA, bool TB, int TAG, bool F32, typename TI, typename TO>
 static void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda, int64_t sA,
                    const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc, int64_t sC, TO* D,
-                   int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om) {
+                   int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om, const KSeg& sg) {
   const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   FH_CHECK(tm * tn < (int64_t)1 << 31 && batch < 65536, -3, "gemm grid too large");
   dim3 grid((unsigned)(tm * tn), (unsigned)batch);
@@ -212,13 +223,23 @@ static void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha,
     gn = (int)std::max<int64_t>(1, std::min<int64_t>(tn, kGroupBytes / per));
   }
   constexpr int OCC = BM * BN > 8192 ? 1 : 4;
-  // lane orientation along N only for mapped outputs (the (T) W build, tag 2)
-  if (TAG == 2 && !lanes_along_m(om))
-    hipLaunchKernelGGL((k_gemm<BM, BN, BK, TA, TB, OCC, TAG, F32, TI, TO, false>), grid, dim3(256), 0, s, M, N, K, alpha,
-                       A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, gn, om);
-  else
-    hipLaunchKernelGGL((k_gemm<BM, BN, BK, TA, TB, OCC, TAG, F32, TI, TO, true>), grid, dim3(256), 0, s, M, N, K, alpha,
-                       A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, gn, om);
+#define FH_GO(LM_, SEG_)                                                                                           \
+  hipLaunchKernelGGL((k_gemm<BM, BN, BK, TA, TB, OCC, TAG, F32, TI, TO, LM_, SEG_>), grid, dim3(256), 0, s, M, N, K, \
+                     alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, kchunk, gn, om, sg)
+  if constexpr (TAG == 2) {
+    // the (T) W build: output maps (lane orientation along N when only the columns are unit-stride)
+    // and two-segment K
+    const bool lm = lanes_along_m(om);
+    if (sg.A) {
+      if (lm) FH_GO(true, true); else FH_GO(false, true);
+    } else {
+      if (lm) FH_GO(true, false); else FH_GO(false, false);
+    }
+  } else {
+    FH_CHECK(!sg.A, -3, "gemm: two-segment K is only instantiated for the (T) W build");
+    FH_GO(true, false);
+  }
+#undef FH_GO
   FH_HIP(hipGetLastError());
 }
 
@@ -228,14 +249,14 @@ static void launch(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha,
 template <bool TA, bool TB, int TAG, bool F32, typename TI, typename TO>
 static void pick(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda, int64_t sA,
                  const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc, int64_t sC, TO* D,
-                 int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om) {
+                 int64_t ldd, int64_t sD, int64_t batch, int64_t kchunk, const OutMap& om, const KSeg& sg) {
   const int64_t kl = kchunk > 0 ? kchunk : K;
   if (kl <= 512)
     launch<64, 64, 8, TA, TB, TAG, F32, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD,
-                                                batch, kchunk, om);
+                                                batch, kchunk, om, sg);
   else
     launch<64, 64, 16, TA, TB, TAG, F32, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd,
-                                                 sD, batch, kchunk, om);
+                                                 sD, batch, kchunk, om, sg);
 }
 
 // split-K sizing: few output tiles and a long K leave most CUs idle; cut K into slices
@@ -275,14 +296,25 @@ static void check_args(int64_t M, int64_t K, int64_t N, bool ta, bool tb, int64_
   FH_CHECK(!map || (map->rmod > 0 && (beta == 0.0 || C == D)), -3, "gemm: mapped output needs rmod > 0 and C == D");
 }
 
+// the second K segment covers k in [K1, K): its operands must hold K - K1 values along k
+void check_seg(int64_t M, int64_t N, int64_t K, bool ta, bool tb, const KSeg& sg) {
+  if (!sg.A) return;
+  FH_CHECK(sg.B != nullptr && sg.K1 > 0 && sg.K1 < K && sg.K1 % 4 == 0, -3,
+           "gemm: second K segment must start at a positive multiple of 4 below K");
+  const int64_t K2 = K - sg.K1;
+  FH_CHECK(sg.lda >= (ta ? K2 : M) && sg.ldb >= (tb ? N : K2), -3, "gemm: second K segment leading dimension too small");
+}
+
 void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
               int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, const double* C,
               int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD, int64_t batch, int tag, double* work,
-              int64_t work_elems, const OutMap* map, bool f32) {
+              int64_t work_elems, const OutMap* map, bool f32, const KSeg* seg) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
-  check_args(M, K, N, ta, tb, lda, ldb, ldd, beta, C, D, ldc, map);
+  check_args(M, seg ? seg->K1 : K, N, ta, tb, lda, ldb, ldd, beta, C, D, ldc, map);
   const OutMap om = map ? *map : OutMap{};
-  const int64_t sl = map ? 1 : splitk_slices(M, N, K, batch);
+  const KSeg sg = seg ? *seg : KSeg{};
+  check_seg(M, N, K, ta, tb, sg);
+  const int64_t sl = (map || sg.A) ? 1 : splitk_slices(M, N, K, batch);
   const bool split = sl > 1 && work != nullptr && work_elems >= sl * M * N;
   const int64_t kchunk = split ? (K + sl - 1) / sl : 0;
   const int64_t nb = split ? (K + kchunk - 1) / kchunk : batch;
@@ -294,13 +326,13 @@ void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, 
   do {                                                                                                              \
     if (tag == 1)                                                                                                   \
       pick<TA_, TB_, 1, F_, double, double>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, \
-                                            kchunk, om);                                                            \
+                                            kchunk, om, sg);                                                        \
     else if (tag == 2)                                                                                              \
       pick<TA_, TB_, 2, F_, double, double>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, \
-                                            kchunk, om);                                                            \
+                                            kchunk, om, sg);                                                        \
     else                                                                                                            \
       pick<TA_, TB_, 0, F_, double, double>(s, M, N, K, al, A, lda, sA, B, ldb, sB, be, Cc, ldc, sC, Dd, lddd, sDD, nb, \
-                                            kchunk, om);                                                            \
+                                            kchunk, om, sg);                                                        \
   } while (0)
 #define FH_PICK(TA_, TB_)              \
   do {                                 \
@@ -326,12 +358,15 @@ void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, 
 template <int TAG, typename TI, typename TO>
 static void gemm_nn_f32(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda,
                         int64_t sA, const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc,
-                        int64_t sC, TO* D, int64_t ldd, int64_t sD, int64_t batch, const OutMap* map) {
+                        int64_t sC, TO* D, int64_t ldd, int64_t sD, int64_t batch, const OutMap* map,
+                        const KSeg* seg = nullptr) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
-  check_args(M, K, N, false, false, lda, ldb, ldd, beta, C, D, ldc, map);
+  check_args(M, seg ? seg->K1 : K, N, false, false, lda, ldb, ldd, beta, C, D, ldc, map);
   const OutMap om = map ? *map : OutMap{};
+  const KSeg sg = seg ? *seg : KSeg{};
+  check_seg(M, N, K, false, false, sg);
   pick<false, false, TAG, true, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch,
-                                        0, om);
+                                        0, om, sg);
 }
 
 void gemm_f32_fd(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const float* A, int64_t lda,
@@ -347,9 +382,9 @@ void gemm_f32_df(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, c
 
 void gemm_f32_ff(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const float* A, int64_t lda, int64_t sA,
                  const float* B, int64_t ldb, int64_t sB, double beta, const float* C, int64_t ldc, int64_t sC, float* D,
-                 int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map) {
-  if (tag == 2) gemm_nn_f32<2>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, map);
-  else gemm_nn_f32<0>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, map);
+                 int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map, const KSeg* seg) {
+  if (tag == 2) gemm_nn_f32<2>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, map, seg);
+  else gemm_nn_f32<0>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, map, seg);
 }
 
 // FP32 operands and output, any transposition (the T2-guess network's layers; tag 3)
@@ -359,7 +394,8 @@ void gemm_f32(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, 
   check_args(M, K, N, ta, tb, lda, ldb, ldc, beta, C, C, ldc, nullptr);
   const OutMap om{};
 #define FH_F32(TA_, TB_) \
-  pick<TA_, TB_, 3, true, float, float>(s, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, C, ldc, 0, 1, 0, om)
+  pick<TA_, TB_, 3, true, float, float>(s, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, C, ldc, 0, 1, 0, om, \
+                                        KSeg{})
   if (ta) {
     if (tb) FH_F32(true, true); else FH_F32(true, false);
   } else {

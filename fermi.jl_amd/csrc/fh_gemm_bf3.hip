@@ -101,12 +101,13 @@ __device__ inline V4 load4(const TI* p, bool full, int n_ok) {
 // read 4 rows x BK/8 k per thread and transposed in registers; "k-contiguous" operands are read
 // 4 k of one row per load.  BK = 32 (two workgroups per CU) for long K, BK = 16 (three per CU,
 // more overlap of one tile's epilogue with another's loop) for short K.
-template <int BK, bool TA, bool TB, int TAG, typename TI, typename TO, bool VEC, bool LM>
+template <int BK, bool TA, bool TB, int TAG, typename TI, typename TO, bool VEC, bool LM, bool SEG>
 __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, int64_t N, int64_t K, double alpha,
                                                      const TI* __restrict__ A, int64_t lda, int64_t sA,
                                                      const TI* __restrict__ B, int64_t ldb, int64_t sB, double beta,
                                                      const TO* C, int64_t ldc, int64_t sC, TO* D, int64_t ldd,
-                                                     int64_t sD, int tilesM, int tilesN, int gn, OutMap om) {
+                                                     int64_t sD, int tilesM, int tilesN, int gn, OutMap om,
+                                                     KSeg sg) {
   constexpr int kRow = BK + 8;   // LDS row (bf16)
   constexpr int KG = BK / 8;      // k per thread (row-contiguous) = loads per thread (both patterns)
   constexpr int KQ = BK / 4;      // 4-wide k groups (k-contiguous)
@@ -138,6 +139,19 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
   B += bz * sB;
   if (C) C += bz * sC;
   D += bz * sD;
+  // second K segment: for k >= K1 the operands are A2 / B2 at k - K1 (K1 % 4 == 0, so no 4-wide
+  // k group straddles the boundary)
+  const TI* A2 = SEG ? static_cast<const TI*>(sg.A) + bz * sg.sA : A;
+  const TI* B2 = SEG ? static_cast<const TI*>(sg.B) + bz * sg.sB : B;
+  const int64_t K1 = SEG ? sg.K1 : K, lda2 = SEG ? sg.lda : lda, ldb2 = SEG ? sg.ldb : ldb;
+  auto aptr = [&](int64_t gm, int64_t gk) {
+    return (!SEG || gk < K1) ? (TA ? A + gk + lda * gm : A + gm + lda * gk)
+                             : (TA ? A2 + (gk - K1) + lda2 * gm : A2 + gm + lda2 * (gk - K1));
+  };
+  auto bptr = [&](int64_t gn_, int64_t gk) {
+    return (!SEG || gk < K1) ? (TB ? B + gn_ + ldb * gk : B + gk + ldb * gn_)
+                             : (TB ? B2 + gn_ + ldb2 * (gk - K1) : B2 + (gk - K1) + ldb2 * gn_);
+  };
 
   // per-thread load coordinates
   // row-contiguous pattern: rows 4*(tid&31)..+3, k KG*(tid>>5)..+KG-1 (one load per k)
@@ -150,7 +164,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
 #pragma unroll
       for (int j = 0; j < KG; ++j) {
         const int64_t gk = k0 + KG * (tid >> 5) + j;
-        ra[j] = load4<TI, VEC>(A + gm + lda * gk, n_ok == 4 && gk < K, gk < K ? n_ok : 0);
+        ra[j] = load4<TI, VEC>(aptr(gm, gk), n_ok == 4 && gk < K, gk < K ? n_ok : 0);
       }
     } else {
       const int64_t gk = k0 + 4 * (tid % KQ);
@@ -158,7 +172,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
 #pragma unroll
       for (int j = 0; j < KG; ++j) {
         const int64_t gm = m0 + tid / KQ + RS * j;
-        ra[j] = load4<TI, VEC>(A + gk + lda * gm, k_ok == 4 && gm < M, gm < M ? k_ok : 0);
+        ra[j] = load4<TI, VEC>(aptr(gm, gk), k_ok == 4 && gm < M, gm < M ? k_ok : 0);
       }
     }
     if (TB) {
@@ -167,7 +181,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
 #pragma unroll
       for (int j = 0; j < KG; ++j) {
         const int64_t gk = k0 + KG * (tid >> 5) + j;
-        rb[j] = load4<TI, VEC>(B + gn_ + ldb * gk, n_ok == 4 && gk < K, gk < K ? n_ok : 0);
+        rb[j] = load4<TI, VEC>(bptr(gn_, gk), n_ok == 4 && gk < K, gk < K ? n_ok : 0);
       }
     } else {
       const int64_t gk = k0 + 4 * (tid % KQ);
@@ -175,7 +189,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
 #pragma unroll
       for (int j = 0; j < KG; ++j) {
         const int64_t gcol = n0 + tid / KQ + RS * j;
-        rb[j] = load4<TI, VEC>(B + gk + ldb * gcol, k_ok == 4 && gcol < N, gcol < N ? k_ok : 0);
+        rb[j] = load4<TI, VEC>(bptr(gcol, gk), k_ok == 4 && gcol < N, gcol < N ? k_ok : 0);
       }
     }
   };
@@ -295,7 +309,7 @@ static bool aligned4(const TI* p, int64_t ld, int64_t stride) {
 template <bool TA, bool TB, int TAG, typename TI, typename TO>
 static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda,
                        int64_t sA, const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc,
-                       int64_t sC, TO* D, int64_t ldd, int64_t sD, int64_t batch, const OutMap& om) {
+                       int64_t sC, TO* D, int64_t ldd, int64_t sD, int64_t batch, const OutMap& om, const KSeg& sg) {
   const int64_t tm = (M + kBM - 1) / kBM, tn = (N + kBN - 1) / kBN;
   FH_CHECK(tm * tn < (int64_t)1 << 31 && batch < 65536, -3, "gemm grid too large");
   dim3 grid((unsigned)(tm * tn), (unsigned)batch);
@@ -304,23 +318,30 @@ static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double al
     const int64_t per = std::max<int64_t>(1, K * kBN * (int64_t)sizeof(TI));
     gn = (int)std::max<int64_t>(1, std::min<int64_t>(tn, kGroupBytesBf3 / per));
   }
-  const bool vec = aligned4(A, lda, batch > 1 ? sA : 0) && aligned4(B, ldb, batch > 1 ? sB : 0);
+  bool vec = aligned4(A, lda, batch > 1 ? sA : 0) && aligned4(B, ldb, batch > 1 ? sB : 0);
+  if (sg.A)
+    vec = vec && aligned4(static_cast<const TI*>(sg.A), sg.lda, batch > 1 ? sg.sA : 0) &&
+          aligned4(static_cast<const TI*>(sg.B), sg.ldb, batch > 1 ? sg.sB : 0);
   // lanes along M unless only the output's column axis has unit stride (mapped (T) outputs)
   const bool lm = om.rmod <= 0 || om.rs0 == 1 || om.cs0 != 1;
-#define FH_BF3_GO(BK_, V_, L_)                                                                                        \
-  hipLaunchKernelGGL((k_gemm_bf3<BK_, TA, TB, TAG, TI, TO, V_, L_>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, \
-                     B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, gn, om)
-  if (TAG == 2) {
-    // the (T) W build: K = v + o is short
-    if (lm) {
-      if (vec) FH_BF3_GO(16, true, true); else FH_BF3_GO(16, false, true);
+#define FH_BF3_GO(BK_, V_, L_, S_)                                                                                    \
+  hipLaunchKernelGGL((k_gemm_bf3<BK_, TA, TB, TAG, TI, TO, V_, L_, S_>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, \
+                     sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, gn, om, sg)
+  if constexpr (TAG == 2) {
+    // the (T) W build: K = 2 (v + o) in two segments, output maps
+    FH_CHECK(sg.A != nullptr, -3, "gemm_bf3: the (T) W build takes two K segments");
+    if (vec) {
+      if (lm) FH_BF3_GO(16, true, true, true); else FH_BF3_GO(16, true, false, true);
     } else {
-      if (vec) FH_BF3_GO(16, true, false); else FH_BF3_GO(16, false, false);
+      if (lm) FH_BF3_GO(16, false, true, true); else FH_BF3_GO(16, false, false, true);
     }
-  } else if (K <= 512) {
-    if (vec) FH_BF3_GO(16, true, true); else FH_BF3_GO(16, false, true);
   } else {
-    if (vec) FH_BF3_GO(32, true, true); else FH_BF3_GO(32, false, true);
+    FH_CHECK(!sg.A, -3, "gemm_bf3: two-segment K is only instantiated for the (T) W build");
+    if (K <= 512) {
+      if (vec) FH_BF3_GO(16, true, true, false); else FH_BF3_GO(16, false, true, false);
+    } else {
+      if (vec) FH_BF3_GO(32, true, true, false); else FH_BF3_GO(32, false, true, false);
+    }
   }
 #undef FH_BF3_GO
   FH_HIP(hipGetLastError());
@@ -329,42 +350,49 @@ static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double al
 template <typename TI, typename TO>
 void gemm_bf3(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda,
               int64_t sA, const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc, int64_t sC,
-              TO* D, int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map) {
+              TO* D, int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map, const KSeg* seg) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   FH_CHECK(K >= 0, -3, "gemm: negative K");
-  FH_CHECK(lda >= (ta ? (K > 0 ? K : 1) : M) && ldb >= (tb ? N : (K > 0 ? K : 1)) && (map || ldd >= M), -3,
+  const int64_t K1 = seg && seg->A ? seg->K1 : K;   // extent of the first K segment
+  FH_CHECK(lda >= (ta ? (K1 > 0 ? K1 : 1) : M) && ldb >= (tb ? N : (K1 > 0 ? K1 : 1)) && (map || ldd >= M), -3,
            "gemm: leading dimension too small");
   FH_CHECK(beta == 0.0 || (C != nullptr && (map || ldc >= M)), -3, "gemm: C missing");
   FH_CHECK(!map || (map->rmod > 0 && (beta == 0.0 || C == D)), -3, "gemm: mapped output needs rmod > 0 and C == D");
   const OutMap om = map ? *map : OutMap{};
-  // instantiated variants: FP64 operands (any transposition, tag 0) for the contraction engine;
-  // FP32 operands, no transposition, for the PP ladder (tag 1, FP64 output) and the (T) W build
-  // (tag 2, FP32 output)
-  if constexpr (std::is_same<TI, double>::value) {
-    FH_CHECK(tag == 0, -3, "gemm_bf3: FP64 operands take tag 0");
-    if (ta) {
-      if (tb) launch_bf3<true, true, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om);
-      else launch_bf3<true, false, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om);
-    } else {
-      if (tb) launch_bf3<false, true, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om);
-      else launch_bf3<false, false, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om);
+  const KSeg sg = seg ? *seg : KSeg{};
+  check_seg(M, N, K, ta, tb, sg);
+  // instantiated variants: FP32 operands with FP64 output, any transposition, tag 0 (the
+  // contraction engine); FP32 operands, no transposition, for the PP ladder (tag 1, FP64 output)
+  // and the (T) W build (tag 2, FP32 output)
+  constexpr bool kOutF64 = std::is_same<TO, double>::value;
+  if (tag == 0) {
+    FH_CHECK(kOutF64, -3, "gemm_bf3: tag 0 has FP64 output");
+    if constexpr (kOutF64) {
+      if (ta) {
+        if (tb) launch_bf3<true, true, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om, sg);
+        else launch_bf3<true, false, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om, sg);
+      } else {
+        if (tb) launch_bf3<false, true, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om, sg);
+        else launch_bf3<false, false, 0, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om, sg);
+      }
     }
+    return;
+  }
+  if constexpr (std::is_same<TI, float>::value) {
+    FH_CHECK(!ta && !tb, -3, "gemm_bf3: tags 1 and 2 take untransposed operands");
+    constexpr int kTag = kOutF64 ? 1 : 2;
+    FH_CHECK(tag == kTag, -3, "gemm_bf3: tag 1 has FP64 output, tag 2 FP32 output");
+    launch_bf3<false, false, kTag, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om, sg);
   } else {
-    FH_CHECK(!ta && !tb, -3, "gemm_bf3: FP32 operands are not transposed");
-    constexpr int kTag = std::is_same<TO, double>::value ? 1 : 2;
-    FH_CHECK(tag == kTag, -3, "gemm_bf3: FP32 operands take tag 1 (FP64 output) or 2 (FP32 output)");
-    launch_bf3<false, false, kTag, TI, TO>(s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch, om);
+    FH_CHECK(false, -3, "gemm_bf3: FP64 operands take tag 0");
   }
 }
 
 template void gemm_bf3<float, float>(hipStream_t, bool, bool, int64_t, int64_t, int64_t, double, const float*, int64_t,
                                      int64_t, const float*, int64_t, int64_t, double, const float*, int64_t, int64_t,
-                                     float*, int64_t, int64_t, int64_t, int, const OutMap*);
+                                     float*, int64_t, int64_t, int64_t, int, const OutMap*, const KSeg*);
 template void gemm_bf3<float, double>(hipStream_t, bool, bool, int64_t, int64_t, int64_t, double, const float*,
                                       int64_t, int64_t, const float*, int64_t, int64_t, double, const double*, int64_t,
-                                      int64_t, double*, int64_t, int64_t, int64_t, int, const OutMap*);
-template void gemm_bf3<double, double>(hipStream_t, bool, bool, int64_t, int64_t, int64_t, double, const double*,
-                                       int64_t, int64_t, const double*, int64_t, int64_t, double, const double*,
-                                       int64_t, int64_t, double*, int64_t, int64_t, int64_t, int, const OutMap*);
+                                      int64_t, double*, int64_t, int64_t, int64_t, int, const OutMap*, const KSeg*);
 
 }  // namespace fh

@@ -87,13 +87,25 @@ struct OutMap {
   int64_t rmod = 0, rs0 = 0, rs1 = 0, cmod = 0, cs0 = 0, cs1 = 0;
 };
 
+// Second K segment of a two-segment GEMM, D = alpha (A B + A2 B2) + beta C: for k >= K1 the
+// operands are A2 / B2 at k - K1 (same transposition, their own leading dimensions and batch
+// strides).  K1 must be a multiple of 4.  A == nullptr: one segment.
+struct KSeg {
+  const void* A = nullptr;
+  int64_t lda = 0, sA = 0;
+  const void* B = nullptr;
+  int64_t ldb = 0, sB = 0;
+  int64_t K1 = 0;
+};
+void check_seg(int64_t M, int64_t N, int64_t K, bool ta, bool tb, const KSeg& sg);
+
 // D = alpha * op(A) * op(B) + beta * C  (column-major, FP64 MFMA); C may alias D.
 // Strided batches: A_i = A + i*sA etc.
 void gemm_f64(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha,
               const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB,
               double beta, const double* C, int64_t ldc, int64_t sC, double* D, int64_t ldd, int64_t sD,
               int64_t batch = 1, int tag = 0, double* work = nullptr, int64_t work_elems = 0,
-              const OutMap* map = nullptr, bool f32 = false);
+              const OutMap* map = nullptr, bool f32 = false, const KSeg* seg = nullptr);
 // f32 = true: operands rounded to FP32 in LDS, FP32 MFMA with FP32 accumulation (mixed precision)
 
 // FP32-MFMA GEMMs (NN) with FP32 data in HBM: _fd float operands -> double output,
@@ -104,16 +116,17 @@ void gemm_f32_df(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, c
                  const double* B, int64_t ldb, double beta, const float* C, int64_t ldc, float* D, int64_t ldd);
 void gemm_f32_ff(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const float* A, int64_t lda, int64_t sA,
                  const float* B, int64_t ldb, int64_t sB, double beta, const float* C, int64_t ldc, int64_t sC, float* D,
-                 int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map);
+                 int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map, const KSeg* seg = nullptr);
 // FP32 operands and output, any transposition, C = alpha op(A) op(B) + beta C (tag 3)
 void gemm_f32(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const float* A,
               int64_t lda, const float* B, int64_t ldb, double beta, float* C, int64_t ldc);
-// split-BF16x3 MFMA GEMM (three bf16 products per FP32 product, FP32 accumulation); TI in
-// {float, double}, TO in {float, double}; tags 0/1/2 give distinct kernel symbols as above
+// split-BF16x3 MFMA GEMM (three bf16 products per FP32 product, FP32 accumulation) on FP32
+// operands: tag 0 any transposition with FP64 output, tag 1 (FP64 output) and tag 2 (FP32 output)
+// untransposed; the tags give distinct kernel symbols as above
 template <typename TI, typename TO>
 void gemm_bf3(hipStream_t s, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda,
               int64_t sA, const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc, int64_t sC,
-              TO* D, int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map);
+              TO* D, int64_t ldd, int64_t sD, int64_t batch, int tag, const OutMap* map, const KSeg* seg = nullptr);
 // ---------------------------------------------------------------- T2-guess network (fh_mlp.hip)
 struct MlpOffsets {
   size_t W1, b1, W2, b2, W3, b3, w4, b4;

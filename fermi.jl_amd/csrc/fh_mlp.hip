@@ -45,31 +45,29 @@ __global__ void k_bias_tanh(int h, int64_t n, const float* __restrict__ b, float
     A[e] = tanhf(A[e] + b[e % h]);
 }
 
-// y[s] = b4 + sum_r w4[r] A[r,s]  (one sample per thread; h values of a sample are contiguous)
-__global__ void k_out(int h, int64_t B, const float* __restrict__ w4, const float* __restrict__ b4,
-                      const float* __restrict__ A, float* __restrict__ y) {
-  for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < B; s += (int64_t)gridDim.x * 256) {
-    const float* a = A + (int64_t)h * s;
-    float t = 0.0f;
-    for (int r = 0; r < h; ++r) t += w4[r] * a[r];
-    y[s] = t + b4[0];
+// y[s] += b4  (after y = w4^T A3 as a GEMM with M = 1)
+__global__ void k_add_scalar(int64_t n, const float* __restrict__ b, float* __restrict__ y) {
+  for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < n; s += (int64_t)gridDim.x * 256) y[s] += b[0];
+}
+
+// dZ3[r,s] = w4[r] dy[s] (1 - A3[r,s]^2), element order of [h,B] (coalesced)
+__global__ void k_out_dz(int h, int64_t n, const float* __restrict__ w4, const float* __restrict__ A3,
+                         const float* __restrict__ dy, float* __restrict__ dZ3) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const float a = A3[e];
+    dZ3[e] = w4[e % h] * dy[e / h] * (1.0f - a * a);
   }
 }
 
-// dy[s] = (y[s] - t[s]) / B;  dZ3[r,s] = w4[r] dy[s] (1 - A3[r,s]^2);  per-block partial loss sums
-__global__ void k_out_back(int h, int64_t B, const float* __restrict__ w4, const float* __restrict__ A3,
-                           const float* __restrict__ y, const float* __restrict__ t, float* __restrict__ dy,
-                           float* __restrict__ dZ3, double* __restrict__ lpart) {
+// dy[s] = (y[s] - t[s]) / B;  per-block partial sums of (y - t)^2 / 2
+__global__ void k_out_back(int64_t B, const float* __restrict__ y, const float* __restrict__ t, float* __restrict__ dy,
+                           double* __restrict__ lpart) {
   __shared__ double red[256];
   double l = 0.0;
   for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < B; s += (int64_t)gridDim.x * 256) {
     const float r = y[s] - t[s];
     l += 0.5 * (double)r * (double)r;
-    const float g = r / (float)B;
-    dy[s] = g;
-    const float* a = A3 + (int64_t)h * s;
-    float* z = dZ3 + (int64_t)h * s;
-    for (int q = 0; q < h; ++q) z[q] = w4[q] * g * (1.0f - a[q] * a[q]);
+    dy[s] = r / (float)B;
   }
   red[threadIdx.x] = l;
   __syncthreads();
@@ -131,7 +129,8 @@ void mlp_forward(hipStream_t s, int h, const float* P, int64_t B, const float* X
   hipLaunchKernelGGL(k_bias_tanh, dim3(grid1(h * B)), dim3(256), 0, s, h, (int64_t)h * B, P + o.b2, A2);
   gemm_f32(s, false, false, h, B, h, 1.0, P + o.W3, h, A2, h, 0.0, A3, h);
   hipLaunchKernelGGL(k_bias_tanh, dim3(grid1(h * B)), dim3(256), 0, s, h, (int64_t)h * B, P + o.b3, A3);
-  hipLaunchKernelGGL(k_out, dim3(grid1(B)), dim3(256), 0, s, h, B, P + o.w4, P + o.b4, A3, y);
+  gemm_f32(s, false, false, 1, B, h, 1.0, P + o.w4, 1, A3, h, 0.0, y, 1);
+  hipLaunchKernelGGL(k_add_scalar, dim3(grid1(B)), dim3(256), 0, s, B, P + o.b4, y);
   FH_HIP(hipGetLastError());
 }
 
@@ -140,8 +139,9 @@ void mlp_backward(hipStream_t s, int h, const float* P, int64_t B, const float* 
                   float* part, double* lpart, int64_t* nlpart) {
   const MlpOffsets o = mlp_offsets(h);
   const unsigned gb = grid1(B);
-  hipLaunchKernelGGL(k_out_back, dim3(gb), dim3(256), 0, s, h, B, P + o.w4, A3, y, T, dy, dZ, lpart);
+  hipLaunchKernelGGL(k_out_back, dim3(gb), dim3(256), 0, s, B, y, T, dy, lpart);
   *nlpart = gb;
+  hipLaunchKernelGGL(k_out_dz, dim3(grid1(h * B)), dim3(256), 0, s, h, (int64_t)h * B, P + o.w4, A3, dy, dZ);
   // output layer: g w4 = A3 dy, g b4 = sum dy
   gemm_f32(s, false, false, h, 1, B, 1.0, A3, h, dy, B, 0.0, G + o.w4, h);
   rowsum(s, 1, B, dy, part, G + o.b4);

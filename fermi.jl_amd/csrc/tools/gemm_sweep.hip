@@ -10,8 +10,8 @@ template <int BM, int BN, int BK, int OCC, bool F32, typename TI>
 double run(int64_t M, int64_t N, int64_t K, const TI* A, const TI* B, double* D, int reps, int gn) {
   const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   hipEvent_t a, b; hipEventCreate(&a); hipEventCreate(&b);
-  auto go = [&]() { hipLaunchKernelGGL((k_gemm<BM, BN, BK, false, false, OCC, 9, F32, TI, double, true>), dim3(tm * tn, 1), dim3(256), 0, 0,
-                     M, N, K, 1.0, A, M, 0, B, K, 0, 0.0, nullptr, M, 0, D, M, 0, (int)tm, (int)tn, (int64_t)0, gn, OutMap{}); };
+  auto go = [&]() { hipLaunchKernelGGL((k_gemm<BM, BN, BK, false, false, OCC, 9, F32, TI, double, true, false>), dim3(tm * tn, 1), dim3(256), 0, 0,
+                     M, N, K, 1.0, A, M, 0, B, K, 0, 0.0, nullptr, M, 0, D, M, 0, (int)tm, (int)tn, (int64_t)0, gn, OutMap{}, KSeg{}); };
   go(); hipDeviceSynchronize();
   hipEventRecord(a);
   for (int r = 0; r < reps; ++r) go();

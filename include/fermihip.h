@@ -48,7 +48,7 @@ typedef struct fh_cc_opts {
    * per FP32 product) until |dE| < bf3_e_switch and rms < bf3_rms_switch, then as in mode 1.
    * A reduced-precision phase also ends when its rms has not improved for 4 iterations.
    * polish_iters < 0 iterates in FP64 to the cc_e_conv / cc_max_rms criteria, polish_iters >= 0
-   * stops after that many FP64 iterations.  The DIIS subspace is kept across switches. */
+   * stops after that many FP64 iterations.  Every precision switch restarts the DIIS subspace. */
   int precision_mode;
   int polish_iters;
   double mixed_e_switch;    /* default 1e-9 */
