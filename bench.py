@@ -119,12 +119,14 @@ def main():
                          "or mp2")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--diis-prec", default="single", choices=["single", "double"])
-    ap.add_argument("--precision", default="mixed", choices=["double", "mixed", "bf16x3"],
+    ap.add_argument("--precision", default="bf16x3", choices=["double", "mixed", "bf16x3"],
                     help="mixed: FP32-MFMA contractions + FP64 polish and FP32 (T) W build; bf16x3: split-BF16x3 "
-                         "contractions, then FP32, then FP64 polish, and a split-BF16x3 (T) W build.  Reduced-"
-                         "precision runs are checked against an FP64 run of the same workload (error_vs_fp64)")
-    ap.add_argument("--polish-iters", type=int, default=-1,
+                         "contractions + FP64 polish and a split-BF16x3 (T) W build.  Reduced-precision runs are "
+                         "checked against the plain FP64 run of the same workload (error_vs_fp64)")
+    ap.add_argument("--polish-iters", type=int, default=2,
                     help="FP64 polish iterations after the reduced-precision phases (-1: to convergence)")
+    ap.add_argument("--bf3-switch", default="1e-7,1e-8", help="|dE|,rms that end the split-BF16x3 phase")
+    ap.add_argument("--mixed-switch", default="1e-9,1e-9", help="|dE|,rms that end the FP32 phase (--precision mixed)")
     ap.add_argument("--verbose", action="store_true", help="print the CC iteration table")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -165,8 +167,11 @@ def main():
     guess_kind = 1 if model is not None else 0
     pair_d = synth.pair_distances(sites_o, pos)
     ctx = fermihip.Context(0)
+    b3e, b3r = (float(t) for t in args.bf3_switch.split(","))
+    mxe, mxr = (float(t) for t in args.mixed_switch.split(","))
     opts = fermihip.Options(diis_prec=args.diis_prec, precision=args.precision, polish_iters=args.polish_iters,
-                            verbose=args.verbose)
+                            verbose=args.verbose, bf3_e_switch=b3e, bf3_rms_switch=b3r, mixed_e_switch=mxe,
+                            mixed_rms_switch=mxr)
     tprec = fermihip.PRECISIONS[args.precision]
 
     def step():
@@ -280,6 +285,7 @@ def main():
                                f"{'learned' if guess_kind else 'MP2'} T2 guess",
                    "o": o, "v": v, "naux": Q, "precision": args.precision, "diis_prec": args.diis_prec,
                    "polish_iters": args.polish_iters, "guess": "learned" if guess_kind else "mp2",
+                   "bf3_switch": args.bf3_switch, "mixed_switch": args.mixed_switch,
                    "parallelism": "single GPU"},
         "energies": {"ccsd_correlation": r.correlation, "triples": Et, "niter": r.niter, "converged": bool(r.converged),
                      "niter_reduced_precision": r.niter_mixed, "niter_bf16x3": r.niter_bf3},

@@ -668,7 +668,7 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   FH_CHECK(op.cc_max_iter >= 0 && op.cc_ndiis >= 1 && op.cc_ndiis <= 8 && op.cc_diis_relax >= 1, -1,
            "invalid CC options");
   FH_CHECK(op.precision_mode >= 0 && op.precision_mode <= 2, -1,
-           "precision_mode must be 0 (FP64), 1 (FP32 then FP64) or 2 (BF16x3, FP32, then FP64)");
+           "precision_mode must be 0 (FP64), 1 (FP32 then FP64) or 2 (split-BF16x3 then FP64)");
   const int64_t n1 = c.nov(), n2 = c.n2();
   const double epsT = 2.220446049250313e-16;
   double e_conv = op.cc_e_conv, max_rms = op.cc_max_rms;
@@ -678,7 +678,7 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   const double dp = op.cc_damp_ratio;
   std::unique_ptr<DIIS> dm1(new DIIS(c, n1, op.cc_ndiis, f32)), dm2(new DIIS(c, n2, op.cc_ndiis, f32));
   c.iters.clear();
-  // mixed precision: the residual's contractions run at `level` (2 split-BF16x3, 1 FP32,
+  // mixed precision: the residual's contractions run at `level` (2 split-BF16x3 or 1 FP32, then
   // 0 FP64).  A reduced-precision phase ends when |dE| and rms fall below its switch thresholds,
   // or when its rms has not reached a new minimum for kStall iterations (the precision's own
   // noise floor).  Each switch starts a fresh DIIS subspace: the error vectors of the lower
@@ -689,7 +689,7 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   double best_rms = 1e300;
   int polish_done = 0, niter_mixed = 0, niter_bf3 = 0;
   const double sw_e = std::max(op.mixed_e_switch, e_conv), sw_rms = std::max(op.mixed_rms_switch, max_rms);
-  const double sw_e2 = std::max(op.bf3_e_switch, sw_e), sw_rms2 = std::max(op.bf3_rms_switch, sw_rms);
+  const double sw_e2 = std::max(op.bf3_e_switch, e_conv), sw_rms2 = std::max(op.bf3_rms_switch, max_rms);
 
   Buf nT1 = std::move(c.T1), nT2 = std::move(c.T2);     // newT (guess)
   Buf oT1 = dalloc(c, n1), oT2 = dalloc(c, n2);         // T
@@ -701,7 +701,7 @@ static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   while (true) {
     while (level > 0 && ((std::fabs(dE) < (level == 2 ? sw_e2 : sw_e) && rms < (level == 2 ? sw_rms2 : sw_rms)) ||
                          stall >= kStall)) {
-      --level;                                             // next precision level
+      level = 0;                                           // FP64 polish
       dm1.reset(new DIIS(c, n1, op.cc_ndiis, f32));
       dm2.reset(new DIIS(c, n2, op.cc_ndiis, f32));
       dE = 1.0;
@@ -998,8 +998,8 @@ void fh_default_opts(fh_cc_opts* p) {
   p->polish_iters = -1;
   p->mixed_e_switch = 1e-9;
   p->mixed_rms_switch = 1e-9;
-  p->bf3_e_switch = 1e-5;
-  p->bf3_rms_switch = 1e-6;
+  p->bf3_e_switch = 1e-7;
+  p->bf3_rms_switch = 1e-8;
 }
 
 const char* fh_last_error(void) { return g_last_error.c_str(); }

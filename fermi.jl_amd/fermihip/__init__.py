@@ -140,12 +140,12 @@ PRECISIONS = {"double": 0, "mixed": 1, "bf16x3": 2}
 def Options(**kw):
     """Fermi option keys read on this path, with the reference defaults (Options.jl:62-102), plus
     the mixed-precision keys of this backend: precision ("double"; "mixed": FP32 contractions then
-    FP64 polish; "bf16x3": split-BF16x3 contractions, then FP32, then FP64 polish), polish_iters
+    FP64 polish; "bf16x3": split-BF16x3 contractions then FP64 polish), polish_iters
     (-1: polish to convergence), mixed_e_switch, mixed_rms_switch, bf3_e_switch, bf3_rms_switch."""
     o = dict(cc_max_iter=50, cc_e_conv=1e-10, cc_max_rms=1e-10, cc_diis=True, cc_ndiis=3, diis_start=3,
              cc_diis_relax=3, diis_prec="single", cc_damp_ratio=0.0, precision_override=False, verbose=False,
-             precision="double", polish_iters=-1, mixed_e_switch=1e-9, mixed_rms_switch=1e-9, bf3_e_switch=1e-5,
-             bf3_rms_switch=1e-6)
+             precision="double", polish_iters=-1, mixed_e_switch=1e-9, mixed_rms_switch=1e-9, bf3_e_switch=1e-7,
+             bf3_rms_switch=1e-8)
     for k, v in kw.items():
         if k not in o:
             raise FermiException(f"unknown option {k}")

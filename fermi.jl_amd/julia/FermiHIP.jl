@@ -105,7 +105,7 @@ function cc_opts()
     CCOpts(Options.get("cc_max_iter"), Options.get("cc_e_conv"), Options.get("cc_max_rms"), Options.get("cc_diis"),
            Options.get("cc_ndiis"), Options.get("diis_start"), Options.get("cc_diis_relax"), prec == "single" ? 1 : 0,
            Options.get("cc_damp_ratio"), Options.get("precision_override"), 0,
-           0, -1, 1e-9, 1e-9, 1e-5, 1e-6)   # FP64 contractions; the mixed modes are selected by the Python/C callers
+           0, -1, 1e-9, 1e-9, 1e-7, 1e-8)   # FP64 contractions; the mixed modes are selected by the Python/C callers
 end
 
 # RCCSD(moints, aoints, ::RCCSDa) for DF ERIs (RCCSDa.jl:37-44): only the three B blocks and F are needed

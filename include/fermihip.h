@@ -43,18 +43,18 @@ typedef struct fh_cc_opts {
   /* Mixed precision (no counterpart in the reference, whose only knob is all-FP32 CC):
    * precision_mode 0 = FP64 contractions throughout; 1 = the residual's contractions run as
    * FP32 MFMA with FP32 accumulation (amplitudes, denominators, DIIS and energies stay FP64)
-   * until |dE| < mixed_e_switch and rms < mixed_rms_switch, then FP64 polish iterations follow;
-   * 2 = the o^3v^3 and o^2v^4 contractions first run as split-BF16x3 MFMA (three bf16 products
-   * per FP32 product) until |dE| < bf3_e_switch and rms < bf3_rms_switch, then as in mode 1.
-   * A reduced-precision phase also ends when its rms has not improved for 4 iterations.
-   * polish_iters < 0 iterates in FP64 to the cc_e_conv / cc_max_rms criteria, polish_iters >= 0
-   * stops after that many FP64 iterations.  Every precision switch restarts the DIIS subspace. */
+   * until |dE| < mixed_e_switch and rms < mixed_rms_switch; 2 = the o^3v^3 and o^2v^4
+   * contractions run as split-BF16x3 MFMA (three bf16 products per FP32 product, FP32
+   * accumulation; smaller contractions FP32) until |dE| < bf3_e_switch and rms < bf3_rms_switch.
+   * A reduced-precision phase also ends when its rms has not improved for 4 iterations.  FP64
+   * polish iterations follow (a fresh DIIS subspace): polish_iters < 0 iterates in FP64 to the
+   * cc_e_conv / cc_max_rms criteria, polish_iters >= 0 stops after that many FP64 iterations. */
   int precision_mode;
   int polish_iters;
   double mixed_e_switch;    /* default 1e-9 */
   double mixed_rms_switch;  /* default 1e-9 */
-  double bf3_e_switch;      /* default 1e-5 */
-  double bf3_rms_switch;    /* default 1e-6 */
+  double bf3_e_switch;      /* default 1e-7 */
+  double bf3_rms_switch;    /* default 1e-8 */
 } fh_cc_opts;
 
 /* Result of fh_ccsd_solve: the scalar fields of Fermi's RCCSD{T} (RCCSD.jl:61-69),
