@@ -227,18 +227,25 @@ def main():
     # rocprofv3 lists them separately)
     FP32_MFMA_PEAK_TFLOPS = 157.3
     BF16_MFMA_PEAK_TFLOPS = 2500.0
-    launches_per_call = {"ladder": 1, "triples_gemm": 6}
+    launches_per_call = {"ladder": 1, "triples_gemm": 3}
     # FP64/FP32 kernel template arguments: BM, BN, BK, TA, TB, waves-per-EU hint, tag, FP32 MFMA,
-    # operand type, output type; bf16x3 kernel: TA, TB, tag, operand type, output type, vector loads
+    # operand type, output type, lanes along M, two-segment K; bf16x3 kernel: BK, TA, TB, tag,
+    # operand type, output type, vector loads, lanes along M, two-segment K
     variants = {
-        "": ({"ladder": "k_gemm<64, 64, 16, false, false, 4, 1, false, double, double> (PP ladder, FP64 MFMA)",
-              "triples_gemm": "k_gemm<64, 64, 8, false, false, 4, 2, false, double, double> ((T) W build, FP64 MFMA)"},
+        "": ({"ladder": "k_gemm<64, 64, 16, false, false, 4, 1, false, double, double, true, false> (PP ladder, "
+                        "FP64 MFMA)",
+              "triples_gemm": "k_gemm<64, 64, 8, false, false, 4, 2, false, double, double, {true|false}, true> ((T) "
+                              "W build, FP64 MFMA)"},
              FP64_MFMA_PEAK_TFLOPS, "FP64 MFMA dense"),
-        "_fp32": ({"ladder": "k_gemm<64, 64, 16, false, false, 4, 1, true, float, double> (PP ladder, FP32 MFMA)",
-                   "triples_gemm": "k_gemm<64, 64, 8, false, false, 4, 2, true, float, float> ((T) W build, FP32 MFMA)"},
+        "_fp32": ({"ladder": "k_gemm<64, 64, 16, false, false, 4, 1, true, float, double, true, false> (PP ladder, "
+                             "FP32 MFMA)",
+                   "triples_gemm": "k_gemm<64, 64, 8, false, false, 4, 2, true, float, float, {true|false}, true> "
+                                   "((T) W build, FP32 MFMA)"},
                   FP32_MFMA_PEAK_TFLOPS, "FP32 MFMA dense"),
-        "_bf16x3": ({"ladder": "k_gemm_bf3<false, false, 1, float, double, true> (PP ladder, split-BF16x3 MFMA)",
-                     "triples_gemm": "k_gemm_bf3<false, false, 2, float, float, true> ((T) W build, split-BF16x3 MFMA)"},
+        "_bf16x3": ({"ladder": "k_gemm_bf3<32, false, false, 1, float, double, true, true, false> (PP ladder, "
+                               "split-BF16x3 MFMA)",
+                     "triples_gemm": "k_gemm_bf3<16, false, false, 2, float, float, true, {true|false}, true> ((T) W "
+                                     "build, split-BF16x3 MFMA; the two lane orientations)"},
                     BF16_MFMA_PEAK_TFLOPS / 3.0, "BF16 MFMA dense / 3 products (FP32-equivalent)"),
     }
 

@@ -272,34 +272,103 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
   // index 8 (e >> 2) + 4 (l >> 5) + (e & 3) of the 32x32 block; the lane index runs along M (rows
   // of D) when LM, else along N.  Output addresses as in the FP64 kernel (column-major with ldd,
   // or the two-mode OutMap); LM is chosen so that the lanes of a store follow the output's
-  // unit-stride axis.
+  // unit-stride axis.  The map's div/mod is done once per 32-wide block and advanced
+  // incrementally; with beta != 0 (C may alias D: in-place accumulation) every C value of the
+  // thread is loaded before the first store, so the loads overlap instead of each waiting
+  // behind the previous store.
+  auto axis_off = [](int64_t x, int64_t mod, int64_t s0, int64_t s1) {
+    return mod > 0 ? (x % mod) * s0 + (x / mod) * s1 : x * s0;
+  };
+  // offset of base + d (0 <= d < 32) along an axis with map (mod, s0, s1), from base's (q, r)
+  auto axis_step = [](int64_t q, int64_t r, int d, int64_t mod, int64_t s0, int64_t s1) {
+    int64_t rr = r + d, qq = q;
+    while (rr >= mod) { rr -= mod; ++qq; }
+    return rr * s0 + qq * s1;
+  };
+  const bool mapped = om.rmod > 0;
+  // per 32x32 block: the lane axis offset and the register axis' (q, r) start
+  struct Blk {
+    int64_t lane_off, q, r, amod, as0, as1, abase, lrow, lcol, rbase, cbase;
+  };
+  auto block = [&](int i, int j) {
+    Blk b;
+    const int li = lane & 31;
+    b.rbase = m0 + wm * 64 + i * 32;
+    b.cbase = n0 + wn * 64 + j * 32;
+    b.lrow = b.rbase + (LM ? li : 0);
+    b.lcol = b.cbase + (LM ? 0 : li);
+    b.q = b.r = 0;
+    if (!mapped) {
+      b.lane_off = LM ? b.lrow : ldd * b.lcol;
+      b.amod = 0; b.as0 = LM ? ldd : 1; b.as1 = 0;
+    } else if (LM) {
+      b.lane_off = axis_off(b.lrow, om.rmod, om.rs0, om.rs1);
+      b.amod = om.cmod; b.as0 = om.cs0; b.as1 = om.cs1;
+    } else {
+      b.lane_off = axis_off(b.lcol, om.cmod, om.cs0, om.cs1);
+      b.amod = om.rmod; b.as0 = om.rs0; b.as1 = om.rs1;
+    }
+    b.abase = LM ? b.cbase : b.rbase;
+    if (b.amod > 0) { b.q = b.abase / b.amod; b.r = b.abase - b.q * b.amod; }
+    return b;
+  };
+  auto elem = [&](const Blk& b, int e, int64_t& off) {
+    const int ri = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+    const int64_t row = LM ? b.lrow : b.rbase + ri, col = LM ? b.cbase + ri : b.lcol;
+    off = b.lane_off + (b.amod > 0 ? axis_step(b.q, b.r, ri, b.amod, b.as0, b.as1) : (b.abase + ri) * b.as0);
+    return row < M && col < N;
+  };
+  if constexpr (TAG == 2) {
+    // the (T) W build accumulates in place (C == D): per 32x32 block, load 8 of the thread's
+    // values before storing any of them
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < 2; ++j) {
+        const Blk b = block(i, j);
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int li = lane & 31, ri = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        const int64_t row = m0 + wm * 64 + i * 32 + (LM ? li : ri);
-        const int64_t col = n0 + wn * 64 + j * 32 + (LM ? ri : li);
-        if (row < M && col < N) {
-          int64_t off;
-          if (om.rmod > 0) {
-            const int64_t roff = (row % om.rmod) * om.rs0 + (row / om.rmod) * om.rs1;
-            const int64_t coff = om.cmod > 0 ? (col % om.cmod) * om.cs0 + (col / om.cmod) * om.cs1 : col * om.cs0;
-            off = roff + coff;
-          } else {
-            off = row + ldd * col;
+        for (int h = 0; h < 16; h += 8) {
+          float cv[8];
+          int64_t off[8];
+          bool in[8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            in[e] = elem(b, h + e, off[e]);
+            cv[e] = (beta != 0.0 && in[e]) ? (float)C[off[e]] : 0.0f;
           }
-          double val = alpha * (double)acc[i][j][e];
-          if (beta != 0.0) val += beta * (double)(om.rmod > 0 ? C[off] : C[row + ldc * col]);
-          D[off] = (TO)val;
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            if (in[e]) D[off[e]] = (TO)(alpha * (double)acc[i][j][h + e] + beta * (double)cv[e]);
         }
       }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const Blk b = block(i, j);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          int64_t off;
+          if (elem(b, e, off)) {
+            double val = alpha * (double)acc[i][j][e];
+            if (beta != 0.0) {
+              const int li = lane & 31, ri = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+              const int64_t row = m0 + wm * 64 + i * 32 + (LM ? li : ri), col = n0 + wn * 64 + j * 32 + (LM ? ri : li);
+              val += beta * (double)(mapped ? C[off] : C[row + ldc * col]);
+            }
+            D[off] = (TO)val;
+          }
+        }
+      }
+  }
 }
 
 // L2 budget of the B blocks of one raster group (bytes)
-static constexpr int64_t kGroupBytesBf3 = 1 << 20;
+#ifndef FH_BF3_GROUP_BYTES
+#define FH_BF3_GROUP_BYTES (1 << 20)
+#endif
+static constexpr int64_t kGroupBytesBf3 = FH_BF3_GROUP_BYTES;
 
 template <typename TI>
 static bool aligned4(const TI* p, int64_t ld, int64_t stride) {

@@ -352,7 +352,10 @@ __global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int
   __shared__ int slot_of[6];
   const int tid = threadIdx.x;
   const int kk = blockIdx.y, k = k0 + kk;
-  const int ob = blockIdx.x;
+  // consecutive workgroups go to different XCDs: give each XCD a contiguous range of orbits, so
+  // the neighbouring cubes of concurrently resident orbits share the W lines in its L2
+  const int nwg = gridDim.x, w = blockIdx.x;
+  const int ob = (w & 7) * (nwg >> 3) + min(w & 7, nwg & 7) + (w >> 3);
   const double wocc = 2.0 - (i == j ? 1.0 : 0.0) - (j == k ? 1.0 : 0.0);
   const int blk[3] = {orbits[3 * ob], orbits[3 * ob + 1], orbits[3 * ob + 2]};
   // the six orderings of the orbit's blocks as indices into blk: (abc) (acb) (bac) (bca) (cab) (cba)
