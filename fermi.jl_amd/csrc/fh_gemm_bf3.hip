@@ -33,7 +33,7 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 namespace {
 
-constexpr int kBM = 128, kBN = 128;
+constexpr int kBM = 128;
 
 // 4 values of an operand, as FP32
 struct V4 {
@@ -101,7 +101,7 @@ __device__ inline V4 load4(const TI* p, bool full, int n_ok) {
 // read 4 rows x BK/8 k per thread and transposed in registers; "k-contiguous" operands are read
 // 4 k of one row per load.  BK = 32 (two workgroups per CU) for long K, BK = 16 (three per CU,
 // more overlap of one tile's epilogue with another's loop) for short K.
-template <int BK, bool TA, bool TB, int TAG, typename TI, typename TO, bool VEC, bool LM, bool SEG>
+template <int BK, int BN, bool TA, bool TB, int TAG, typename TI, typename TO, bool VEC, bool LM, bool SEG>
 __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, int64_t N, int64_t K, double alpha,
                                                      const TI* __restrict__ A, int64_t lda, int64_t sA,
                                                      const TI* __restrict__ B, int64_t ldb, int64_t sB, double beta,
@@ -112,8 +112,11 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
   constexpr int KG = BK / 8;      // k per thread (row-contiguous) = loads per thread (both patterns)
   constexpr int KQ = BK / 4;      // 4-wide k groups (k-contiguous)
   constexpr int RS = 256 / KQ;    // row step (k-contiguous)
+  constexpr int NJ = BN / 64;     // 32-wide accumulator blocks per wavefront along N
+  constexpr int LB = BN * BK / 1024;  // B loads per thread (k-contiguous)
+  static_assert(BN == 128 || (BN == 64 && !TB), "64-wide N tiles take a k-contiguous B");
   __shared__ __attribute__((aligned(16))) uint16_t As[2][2][kBM * kRow];  // [buf][hi/lo]
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][2][kBN * kRow];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][2][BN * kRow];
 
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wm = wid & 1, wn = wid >> 1;
@@ -133,7 +136,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
     mt = rem / wdt;
     nt = g * gn + rem % wdt;
   }
-  const int64_t m0 = mt * kBM, n0 = nt * kBN;
+  const int64_t m0 = mt * kBM, n0 = nt * BN;
   const int64_t bz = blockIdx.y;
   A += bz * sA;
   B += bz * sB;
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
   // per-thread load coordinates
   // row-contiguous pattern: rows 4*(tid&31)..+3, k KG*(tid>>5)..+KG-1 (one load per k)
   // k-contiguous pattern:  k 4*(tid%KQ)..+3, rows tid/KQ + RS j, j < KG
-  V4 ra[KG], rb[KG];
+  V4 ra[KG], rb[KG > LB ? KG : LB];
   auto gload = [&](int64_t k0) {
     if (!TA) {
       const int64_t gm = m0 + 4 * (tid & 31);
@@ -187,7 +190,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
       const int64_t gk = k0 + 4 * (tid % KQ);
       const int k_ok = clamp4(K - gk);
 #pragma unroll
-      for (int j = 0; j < KG; ++j) {
+      for (int j = 0; j < LB; ++j) {
         const int64_t gcol = n0 + tid / KQ + RS * j;
         rb[j] = load4<TI, VEC>(bptr(gcol, gk), k_ok == 4 && gcol < N, gcol < N ? k_ok : 0);
       }
@@ -206,22 +209,23 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
       }
     }
   };
-  auto put_k = [&](uint16_t* hi, uint16_t* lo, const V4* r) {
+  auto put_k = [&](uint16_t* hi, uint16_t* lo, const V4* r, int nl) {
 #pragma unroll
-    for (int j = 0; j < KG; ++j) put4(hi, lo, (tid / KQ + RS * j) * kRow + 4 * (tid % KQ), r[j]);
+    for (int j = 0; j < KG; ++j)
+      if (j < nl) put4(hi, lo, (tid / KQ + RS * j) * kRow + 4 * (tid % KQ), r[j]);
   };
   auto sstore = [&](int buf) {
     if (!TA) put_rows(As[buf][0], As[buf][1], ra);
-    else put_k(As[buf][0], As[buf][1], ra);
-    if (TB) put_rows(Bs[buf][0], Bs[buf][1], rb);
-    else put_k(Bs[buf][0], Bs[buf][1], rb);
+    else put_k(As[buf][0], As[buf][1], ra, KG);
+    if constexpr (TB) put_rows(Bs[buf][0], Bs[buf][1], rb);
+    else put_k(Bs[buf][0], Bs[buf][1], rb, LB);
   };
 
-  f32x16 acc[2][2];
+  f32x16 acc[2][NJ];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < NJ; ++j)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
 
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
     if (s + 1 < nst) gload((s + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
-      bf16x8 ah[2], al[2], bh[2], bl[2];
+      bf16x8 ah[2], al[2], bh[NJ], bl[NJ];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const int off = (wm * 64 + i * 32 + frow) * kRow + kk * 16 + fk;
@@ -243,15 +247,15 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
         al[i] = *reinterpret_cast<const bf16x8*>(&As[buf][1][off]);
       }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int off = (wn * 64 + j * 32 + frow) * kRow + kk * 16 + fk;
+      for (int j = 0; j < NJ; ++j) {
+        const int off = (wn * (BN / 2) + j * 32 + frow) * kRow + kk * 16 + fk;
         bh[j] = *reinterpret_cast<const bf16x8*>(&Bs[buf][0][off]);
         bl[j] = *reinterpret_cast<const bf16x8*>(&Bs[buf][1][off]);
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < NJ; ++j) {
           // LM: B fragment as the first MFMA operand, so the accumulator lanes run along M
           if constexpr (LM) {
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
@@ -294,7 +298,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
     Blk b;
     const int li = lane & 31;
     b.rbase = m0 + wm * 64 + i * 32;
-    b.cbase = n0 + wn * 64 + j * 32;
+    b.cbase = n0 + wn * (BN / 2) + j * 32;
     b.lrow = b.rbase + (LM ? li : 0);
     b.lcol = b.cbase + (LM ? 0 : li);
     b.q = b.r = 0;
@@ -324,7 +328,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < NJ; ++j) {
         const Blk b = block(i, j);
 #pragma unroll
         for (int h = 0; h < 16; h += 8) {
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < NJ; ++j) {
         const Blk b = block(i, j);
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
@@ -354,7 +358,8 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
             double val = alpha * (double)acc[i][j][e];
             if (beta != 0.0) {
               const int li = lane & 31, ri = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-              const int64_t row = m0 + wm * 64 + i * 32 + (LM ? li : ri), col = n0 + wn * 64 + j * 32 + (LM ? ri : li);
+              const int64_t row = m0 + wm * 64 + i * 32 + (LM ? li : ri),
+                            col = n0 + wn * (BN / 2) + j * 32 + (LM ? ri : li);
               val += beta * (double)(mapped ? C[off] : C[row + ldc * col]);
             }
             D[off] = (TO)val;
@@ -379,12 +384,15 @@ template <bool TA, bool TB, int TAG, typename TI, typename TO>
 static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const TI* A, int64_t lda,
                        int64_t sA, const TI* B, int64_t ldb, int64_t sB, double beta, const TO* C, int64_t ldc,
                        int64_t sC, TO* D, int64_t ldd, int64_t sD, int64_t batch, const OutMap& om, const KSeg& sg) {
-  const int64_t tm = (M + kBM - 1) / kBM, tn = (N + kBN - 1) / kBN;
+  // 64-wide N tiles for the (T) GEMMs whose N is one virtual index (N = v: 128-wide tiles would
+  // leave a partly empty second tile)
+  const int bn = (TAG == 2 && N <= 256) ? 64 : 128;
+  const int64_t tm = (M + kBM - 1) / kBM, tn = (N + bn - 1) / bn;
   FH_CHECK(tm * tn < (int64_t)1 << 31 && batch < 65536, -3, "gemm grid too large");
   dim3 grid((unsigned)(tm * tn), (unsigned)batch);
   int gn = 0;
   if (M > N) {
-    const int64_t per = std::max<int64_t>(1, K * kBN * (int64_t)sizeof(TI));
+    const int64_t per = std::max<int64_t>(1, K * bn * (int64_t)sizeof(TI));
     gn = (int)std::max<int64_t>(1, std::min<int64_t>(tn, kGroupBytesBf3 / per));
   }
   bool vec = aligned4(A, lda, batch > 1 ? sA : 0) && aligned4(B, ldb, batch > 1 ? sB : 0);
@@ -393,17 +401,28 @@ static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double al
           aligned4(static_cast<const TI*>(sg.B), sg.ldb, batch > 1 ? sg.sB : 0);
   // lanes along M unless only the output's column axis has unit stride (mapped (T) outputs)
   const bool lm = om.rmod <= 0 || om.rs0 == 1 || om.cs0 != 1;
-#define FH_BF3_GO(BK_, V_, L_, S_)                                                                                    \
-  hipLaunchKernelGGL((k_gemm_bf3<BK_, TA, TB, TAG, TI, TO, V_, L_, S_>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, \
-                     sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, gn, om, sg)
-  if constexpr (TAG == 2) {
+#define FH_BF3_GO(BK_, V_, L_, S_) FH_BF3_GO_N(BK_, 128, V_, L_, S_)
+#define FH_BF3_GO_N(BK_, BN_, V_, L_, S_)                                                                            \
+  hipLaunchKernelGGL((k_gemm_bf3<BK_, BN_, TA, TB, TAG, TI, TO, V_, L_, S_>), grid, dim3(256), 0, s, M, N, K, alpha, A, \
+                     lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, (int)tm, (int)tn, gn, om, sg)
+  if constexpr (TAG == 2 && !TB) {
     // the (T) W build: K = 2 (v + o) in two segments, output maps
     FH_CHECK(sg.A != nullptr, -3, "gemm_bf3: the (T) W build takes two K segments");
-    if (vec) {
-      if (lm) FH_BF3_GO(16, true, true, true); else FH_BF3_GO(16, true, false, true);
+    if (bn == 64) {
+      if (vec) {
+        if (lm) FH_BF3_GO_N(16, 64, true, true, true); else FH_BF3_GO_N(16, 64, true, false, true);
+      } else {
+        if (lm) FH_BF3_GO_N(16, 64, false, true, true); else FH_BF3_GO_N(16, 64, false, false, true);
+      }
     } else {
-      if (lm) FH_BF3_GO(16, false, true, true); else FH_BF3_GO(16, false, false, true);
+      if (vec) {
+        if (lm) FH_BF3_GO(16, true, true, true); else FH_BF3_GO(16, true, false, true);
+      } else {
+        if (lm) FH_BF3_GO(16, false, true, true); else FH_BF3_GO(16, false, false, true);
+      }
     }
+  } else if constexpr (TAG == 2) {
+    FH_CHECK(false, -3, "gemm_bf3: the (T) W build takes an untransposed B");
   } else {
     FH_CHECK(!sg.A, -3, "gemm_bf3: two-segment K is only instantiated for the (T) W build");
     if (K <= 512) {
@@ -413,6 +432,7 @@ static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double al
     }
   }
 #undef FH_BF3_GO
+#undef FH_BF3_GO_N
   FH_HIP(hipGetLastError());
 }
 
