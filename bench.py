@@ -22,8 +22,12 @@ for _p in (ROOT, os.path.join(ROOT, "fermi.jl_amd")):
 FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
-PREC_NAMES = {"double": "FP64", "mixed": "mixed FP32/FP64",
-              "bf16x3": "mixed split-BF16x3/FP32/FP64"}
+PREC_NAMES = {"double": "FP64", "mixed": "FP32 MFMA + FP64 polish",
+              "bf16x3": "split-BF16x3 MFMA + FP64 polish"}
+
+# HBM bytes per launch of the roofline kernels, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
+# scaled by a plain 1 GiB copy (FETCH x 2.0, WRITE x 1.0): profiles/triples_bf16x3_water10_hbm_counters.md
+MEASURED_TRAFFIC = {("water10", "triples_gemm_bf16x3"): 1.622e9}
 
 
 def log(*a):
@@ -256,7 +260,8 @@ def main():
         names, peak, pname = variants[sfx]
         nl = kd["calls"] * launches_per_call[n]
         return {"kernel": names[n], "bound": "mfma", "achieved": kd["tflops"], "peak": peak, "peak_kind": pname,
-                "unit": "TFLOP/s", "frac": kd["tflops"] / peak, "traffic": None,
+                "unit": "TFLOP/s", "frac": kd["tflops"] / peak,
+                "traffic": MEASURED_TRAFFIC.get((args.config, n + sfx)),
                 "flops_per_launch": kd["flops"] / max(1, nl), "avg_launch_us": kd["ms"] * 1e3 / max(1, nl),
                 "launches": nl}
 
@@ -282,6 +287,7 @@ def main():
                     "niter": r64.niter, "guess": "mp2", "speedup_vs_fp64": t64 / tts,
                     "error_vs_fp64_Eh": {"ccsd": r.correlation - r64.correlation, "triples": Et - Et64,
                                          "total": (r.correlation + Et) - (r64.correlation + Et64)}}
+        fp64_ref["within_1uEh"] = abs(fp64_ref["error_vs_fp64_Eh"]["total"]) <= 1e-6
     out = {
         "metric": "DF-RCCSD(T) time-to-solution (s) + MFMA TFLOP/s at <=1 uEh vs FP64 GPU path",
         "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
