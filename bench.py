@@ -117,7 +117,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="water10", help="water1 | water4 | water10 (synthetic (H2O)n clusters)")
+    ap.add_argument("--config", default="water10",
+                    help="water1 | water4 | water10 (synthetic (H2O)n clusters) | triples64 (the (T)-only synthetic "
+                         "tensors o=64, v=320, naux=1200; with --phase triples)")
     ap.add_argument("--guess", default="auto", choices=["auto", "learned", "mp2"],
                     help="initial T2: learned (the committed network; auto = learned when the model file exists) "
                          "or mp2")
@@ -156,16 +158,25 @@ def main():
         print(json.dumps({"phase": "copy", "bytes_read": n * 8, "bytes_written": n * 8}), flush=True)
         return
 
+    import numpy as np
     import fermihip
     from fermihip import synth
 
     from fermihip import train_guess
-    nw = synth.WATERS[args.config]
-    work = synth.water_cluster_like(nw, args.seed)
-    eo, ev, Boo, Bov, Bvv, sites_o, pos = work
+    if args.config == "triples64":
+        if args.phase != "triples":
+            raise SystemExit("bench.py: the triples64 tensors are a (T)-only workload (use --phase triples)")
+        eo, ev, Boo, Bov, Bvv, _, _ = synth.triples_config(seed=args.seed)
+        nw, sites_o, pos = 0, np.zeros(eo.size, dtype=int), np.zeros((1, 3))
+        work = (eo, ev, Boo, Bov, Bvv, sites_o, pos)
+    else:
+        nw = synth.WATERS[args.config]
+        work = synth.water_cluster_like(nw, args.seed)
+        eo, ev, Boo, Bov, Bvv, sites_o, pos = work
     o, v, Q = Bov.shape[1], Bov.shape[2], Bov.shape[0]
     log(f"bench: config {args.config} ((H2O){nw}) o={o} v={v} naux={Q} seed={args.seed}")
-    model = train_guess.load_model() if args.guess != "mp2" else None
+    # the (T)-only tensors take the MP2 amplitudes as T2
+    model = train_guess.load_model() if args.guess != "mp2" and args.config != "triples64" else None
     if args.guess == "learned" and model is None:
         raise SystemExit("bench.py: --guess learned but no model file (fermi.jl_amd/fermihip/t2_guess_water.npz)")
     guess_kind = 1 if model is not None else 0
@@ -184,7 +195,8 @@ def main():
         if guess_kind == 1:
             ctx.set_t2_model(model, pair_d)    # network + pair distances (untimed, like the upload)
         t1 = time.perf_counter()
-        ctx.prepare()
+        if args.phase != "triples":
+            ctx.prepare()                      # all integral blocks ((T) alone builds what it needs)
         ctx.guess(kind=guess_kind, copy_out=False)
         if args.phase == "triples":
             r = None
@@ -199,8 +211,11 @@ def main():
             dt, up, _, Et = step()
             log(f"bench: (T)-only step {k} {dt:.3f} s Et(MP2 amplitudes)={Et:.12e}")
         st = ctx.stats()["phases"]
-        print(json.dumps({"phase": "triples", "config": args.config, "seconds": dt, "Et": Et,
-                          "triples_gemm": st.get("triples_gemm"), "triples_energy": st.get("triples_energy")}), flush=True)
+        sfx = {"double": "", "mixed": "_fp32", "bf16x3": "_bf16x3"}[args.precision]
+        print(json.dumps({"phase": "triples", "config": args.config, "o": o, "v": v, "naux": Q,
+                          "precision": args.precision, "seconds": dt, "Et": Et,
+                          "triples": st.get("triples" + sfx), "triples_gemm": st.get("triples_gemm" + sfx),
+                          "triples_energy": st.get("triples_energy" + sfx)}), flush=True)
         ctx.close()
         return
 

@@ -354,7 +354,7 @@ static void prepare(Ctx& c) {
   c.lowp = 0;  // integrals are assembled in FP64
   const int64_t o = c.o, v = c.v, Q = c.naux;
   Timer t(c, "integrals", 2.0 * Q * (double)(o * o * o * o + o * o * o * v + o * o * v * v + o * v * o * v +
-                                              o * v * v * v + v * v * v * v));
+                                              o * v * v * v));
   TView Boo = tv(c.Boo.d(), {Q, o, o}), Bov = tv(c.Bov.d(), {Q, o, v}), Bvv = tv(c.Bvv.d(), {Q, v, v});
   c.OOOO = dalloc(c, o * o * o * o);
   c.OOOV = dalloc(c, o * o * o * v);
@@ -372,14 +372,27 @@ static void prepare(Ctx& c) {
   TView ovov = tv(c.OVOV.d(), {o, v, o, v});
   perm(c, ovov, "kcld", tv(c.Lovov.d(), {o, v, o, v}), "kcld", 2.0, 0.0);
   perm(c, ovov, "kdlc", tv(c.Lovov.d(), {o, v, o, v}), "kcld", -1.0, 1.0);
-  // ladder operand Vperm[c,d,a,b] = (ca|db) = Bvv[Q,c,a] Bvv[Q,d,b]  (DFERI.jl:194 permuted)
-  c.Vperm32.reset();  // its FP32 copy is remade from the new Vperm on first use
+  c.Vperm.reset();   // the ladder operands follow on first use (prepare_ladder)
+  c.Vperm32.reset();
+  c.Pt.reset();
+  c.prepared = true;
+  c.lowp = keep;
+}
+
+// The PP ladder's operands (built on the first residual; (T) does not need them):
+// Vperm[c,d,a,b] = (ca|db) = Bvv[Q,c,a] Bvv[Q,d,b] (DFERI.jl:194 permuted), Pt[c,d,a,k] = (kd|ac)
+static void prepare_ladder(Ctx& c) {
+  prepare(c);
+  if (c.Vperm.p) return;
+  const int keep = c.lowp;
+  c.lowp = 0;
+  const int64_t o = c.o, v = c.v, Q = c.naux;
+  Timer t(c, "integrals_vvvv", 2.0 * Q * (double)(v * v * v * v));
+  TView Bvv = tv(c.Bvv.d(), {Q, v, v});
   c.Vperm = dalloc(c, v * v * v * v);
   contract(c, 1.0, Bvv, "Qca", Bvv, "Qdb", 0.0, tv(c.Vperm.d(), {v, v, v, v}), "cdab");
-  // Pt[c,d,a,k] = (kd|ac) = OVVV[k,d,a,c]
   c.Pt = dalloc(c, v * v * v * o);
   perm(c, tv(c.OVVV.d(), {o, v, v, v}), "kdac", tv(c.Pt.d(), {v, v, v, o}), "cdak", 1.0, 0.0);
-  c.prepared = true;
   c.lowp = keep;
 }
 
@@ -387,7 +400,7 @@ static void prepare(Ctx& c) {
 // R1 = numerator of the T1 update (RCCSDHelper.jl:57-85); R2 = numerator of the T2 update
 // (RCCSDHelper.jl:123-220), both without the diagonal Fock term.
 static void residual(Ctx& c, const double* t1p, const double* t2p, double* R1, double* R2) {
-  prepare(c);
+  prepare_ladder(c);
   const int64_t o = c.o, v = c.v;
   const double o_ = (double)o, v_ = (double)v;
   Timer tm(c, "residual", 0.0);
@@ -1065,7 +1078,7 @@ int fh_prepare(fh_ctx* x) {
   return guarded([&] {
     FH_CHECK(x != nullptr, -1, "null context");
     DeviceGuard g(x->c.device);
-    prepare(x->c);
+    prepare_ladder(x->c);
     FH_HIP(hipStreamSynchronize(x->c.s));
     flush_timers(x->c);
   });

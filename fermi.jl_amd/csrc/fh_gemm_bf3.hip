@@ -370,6 +370,10 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
 }
 
 // L2 budget of the B blocks of one raster group (bytes)
+// K tile of the (T) W-build launches
+#ifndef FH_BF3_TBK
+#define FH_BF3_TBK 16
+#endif
 #ifndef FH_BF3_GROUP_BYTES
 #define FH_BF3_GROUP_BYTES (1 << 20)
 #endif
@@ -408,17 +412,18 @@ static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double al
   if constexpr (TAG == 2 && !TB) {
     // the (T) W build: K = 2 (v + o) in two segments, output maps
     FH_CHECK(sg.A != nullptr, -3, "gemm_bf3: the (T) W build takes two K segments");
+    constexpr int TBK = FH_BF3_TBK;
     if (bn == 64) {
       if (vec) {
-        if (lm) FH_BF3_GO_N(16, 64, true, true, true); else FH_BF3_GO_N(16, 64, true, false, true);
+        if (lm) FH_BF3_GO_N(TBK, 64, true, true, true); else FH_BF3_GO_N(TBK, 64, true, false, true);
       } else {
-        if (lm) FH_BF3_GO_N(16, 64, false, true, true); else FH_BF3_GO_N(16, 64, false, false, true);
+        if (lm) FH_BF3_GO_N(TBK, 64, false, true, true); else FH_BF3_GO_N(TBK, 64, false, false, true);
       }
     } else {
       if (vec) {
-        if (lm) FH_BF3_GO(16, true, true, true); else FH_BF3_GO(16, true, false, true);
+        if (lm) FH_BF3_GO(TBK, true, true, true); else FH_BF3_GO(TBK, true, false, true);
       } else {
-        if (lm) FH_BF3_GO(16, false, true, true); else FH_BF3_GO(16, false, false, true);
+        if (lm) FH_BF3_GO(TBK, false, true, true); else FH_BF3_GO(TBK, false, false, true);
       }
     }
   } else if constexpr (TAG == 2) {
