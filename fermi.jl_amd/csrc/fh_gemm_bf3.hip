@@ -15,8 +15,9 @@
 // tile as its first operand, so the accumulator's lane index runs along the rows of D and the
 // epilogue stores are coalesced along the column-major output.
 //
-// Global loads: four consecutive elements per thread (dwordx4 for FP32) when the leading
-// dimensions, batch strides and base addresses allow it (VEC), scalar loads otherwise.
+// Global loads: k-contiguous operands load four consecutive k per thread (dwordx4 for FP32) when
+// the leading dimensions, batch strides and base addresses allow it (VEC), scalar loads
+// otherwise; row-contiguous operands load one element per (row, k), coalesced across lanes.
 // Tile order: the XCD-aware remap and the L2-grouped raster of the FP64/FP32 kernel.
 #include "fh_internal.h"
 
@@ -157,17 +158,21 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
   };
 
   // per-thread load coordinates
-  // row-contiguous pattern: rows 4*(tid&31)..+3, k KG*(tid>>5)..+KG-1 (one load per k)
-  // k-contiguous pattern:  k 4*(tid%KQ)..+3, rows tid/KQ + RS j, j < KG
+  // row-contiguous pattern: rows (tid&31) + 32 i (i < 4), k KG*(tid>>5)..+KG-1; one scalar load
+  //   per (row, k), coalesced across the lanes, and the LDS stores of consecutive lanes land on
+  //   consecutive rows (a row stride of 4 rows would put every lane on the same few banks)
+  // k-contiguous pattern:  k 4*(tid%KQ)..+3, rows tid/KQ + RS j, j < LB
   V4 ra[KG], rb[KG > LB ? KG : LB];
   auto gload = [&](int64_t k0) {
     if (!TA) {
-      const int64_t gm = m0 + 4 * (tid & 31);
-      const int n_ok = clamp4(M - gm);
 #pragma unroll
       for (int j = 0; j < KG; ++j) {
         const int64_t gk = k0 + KG * (tid >> 5) + j;
-        ra[j] = load4<TI, VEC>(aptr(gm, gk), n_ok == 4 && gk < K, gk < K ? n_ok : 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t gm = m0 + (tid & 31) + 32 * i;
+          ra[j].x[i] = (gm < M && gk < K) ? (float)*aptr(gm, gk) : 0.0f;
+        }
       }
     } else {
       const int64_t gk = k0 + 4 * (tid % KQ);
@@ -179,12 +184,14 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
       }
     }
     if (TB) {
-      const int64_t gn_ = n0 + 4 * (tid & 31);
-      const int n_ok = clamp4(N - gn_);
 #pragma unroll
       for (int j = 0; j < KG; ++j) {
         const int64_t gk = k0 + KG * (tid >> 5) + j;
-        rb[j] = load4<TI, VEC>(bptr(gn_, gk), n_ok == 4 && gk < K, gk < K ? n_ok : 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t gn_ = n0 + (tid & 31) + 32 * i;
+          rb[j].x[i] = (gn_ < N && gk < K) ? (float)*bptr(gn_, gk) : 0.0f;
+        }
       }
     } else {
       const int64_t gk = k0 + 4 * (tid % KQ);
@@ -196,11 +203,11 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
       }
     }
   };
-  // r[j].x[i] = X[row 4(tid&31)+i][k KG(tid>>5)+j]: gather the KG k of each row
+  // r[j].x[i] = X[row (tid&31)+32i][k KG(tid>>5)+j]: gather the KG k of each row
   auto put_rows = [&](uint16_t* hi, uint16_t* lo, const V4* r) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int off = (4 * (tid & 31) + i) * kRow + KG * (tid >> 5);
+      const int off = ((tid & 31) + 32 * i) * kRow + KG * (tid >> 5);
       if constexpr (KG == 4) {
         const V4 t = {{r[0].x[i], r[1].x[i], r[2].x[i], r[3].x[i]}};
         put4(hi, lo, off, t);
