@@ -328,126 +328,195 @@ void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, 
 // (the summand Ef of ijk.jl:132 is symmetric in (a,b,c); a=b=c terms vanish), times
 // (2 - d_ij - d_jk).  W[a,b,c] for the k-chunk lies in one buffer with a fastest
 // (W + a + v*b + v^2*c + v^3*kk).  The virtual space is cut into 8x8x8 cubes; one workgroup
-// takes an orbit {A,B,C} (A>=B>=C) of cube blocks and the (up to) six permuted cubes of it.
-// Each thread keeps the W values of its elements in registers and forms
+// takes an orbit {A,B,C} (A>=B>=C) of cube blocks and the (up to) six permuted cubes of it, for
+// a run of consecutive k.  Each thread keeps the W values of its elements in registers and forms
 //     V = W + T1p[a,i] (jb|kc) + (ia|kc) T1p[b,j] + (ia|jb) T1p[c,k]          (ijk.jl:116)
-// from the orbit's T1 vectors and (ia|jb) blocks, staged once in LDS; the V cubes go to LDS
-// (rows padded to 9 and planes to 81 doubles, so the permuted reads of the energy sum spread
-// over the banks), and every element of W is read once per triple.
+// from the orbit's T1 vectors and (ia|jb) blocks staged in LDS (the i, j parts once, the k parts
+// per k); the V cubes go to LDS and every element of W is read once per triple.  The W of the
+// next k is loaded while the energy sum of the current one runs.
 static constexpr int CT = 8, CT3 = CT * CT * CT;
-static constexpr int VP = 648;  // padded cube: x + 9 y + 81 z
 
-__device__ __forceinline__ int vpad(int x, int y, int z) { return x + 9 * y + 81 * z; }
+// LDS layout of one V cube, chosen so that the six permuted reads of the energy sum (each a plane
+// of the cube with one index fixed) spread over the banks.  FP64: rows padded to 9 and planes to
+// 81 elements.  FP32 (V of an FP32 W is kept in FP32): a Latin-square bank swizzle, every
+// x-, y- and z-plane of the cube covers 64 distinct banks.
+template <typename TV>
+struct Cube;
+template <>
+struct Cube<double> {
+  static constexpr int pitch = 648;
+  static __device__ __forceinline__ int off(int x, int y, int z) { return x + 9 * y + 81 * z; }
+};
+template <>
+struct Cube<float> {
+  static constexpr int pitch = 512;
+  static __device__ __forceinline__ int off(int x, int y, int z) { return 64 * z + ((x + z) & 7) + 8 * ((y + z) & 7); }
+};
+
+// index of the ordering (x, y, z) of the orbit's three blocks in (abc) (acb) (bac) (bca) (cab) (cba)
+__device__ constexpr int ord_index(int x, int y) { return x == 0 ? (y == 1 ? 0 : 1) : x == 1 ? (y == 0 ? 2 : 3) : (y == 0 ? 4 : 5); }
+
+// 1/d by the hardware reciprocal and two Newton steps (relative error ~1 ulp; d is a negative
+// orbital-energy denominator, never zero, denormal or infinite)
+__device__ __forceinline__ double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
 
 template <typename TW>
-__global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int j, int k0, const TW* __restrict__ W,
-                                                        const int* __restrict__ orbits,
+__global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int j, int k0, int nk, int kpw,
+                                                        const TW* __restrict__ W, const int* __restrict__ orbits,
                                                         const double* __restrict__ t1p, const double* __restrict__ vvoo,
                                                         const double* __restrict__ eo, const double* __restrict__ ev,
                                                         double* __restrict__ partials) {
-  __shared__ double Vc[6][VP];
-  __shared__ double s_t1[3][3][CT];        // [occupied i/j/k][block A/B/C][8]
+  using TV = TW;
+  __shared__ TV Vc[6][Cube<TV>::pitch];
+  __shared__ double s_t1[3][3][CT];          // [occupied i/j/k][block A/B/C][8]
   __shared__ double s_vv[3][3][3][CT * CT];  // [(jk) (ik) (ij)][block x][block y][x + 8 y]
   __shared__ double s_ev[3][CT];
-  __shared__ int slot_of[6];
   const int tid = threadIdx.x;
-  const int kk = blockIdx.y, k = k0 + kk;
   // consecutive workgroups go to different XCDs: give each XCD a contiguous range of orbits, so
   // the neighbouring cubes of concurrently resident orbits share the W lines in its L2
   const int nwg = gridDim.x, w = blockIdx.x;
   const int ob = (w & 7) * (nwg >> 3) + min(w & 7, nwg & 7) + (w >> 3);
-  const double wocc = 2.0 - (i == j ? 1.0 : 0.0) - (j == k ? 1.0 : 0.0);
+  const int kb = blockIdx.y * kpw, ke = min(nk, kb + kpw);
   const int blk[3] = {orbits[3 * ob], orbits[3 * ob + 1], orbits[3 * ob + 2]};
   // the six orderings of the orbit's blocks as indices into blk: (abc) (acb) (bac) (bca) (cab) (cba)
-  const int pb[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-  if (tid < 6) {
-    int s = tid;
-    for (int q = 0; q < tid; ++q)
-      if (blk[pb[q][0]] == blk[pb[tid][0]] && blk[pb[q][1]] == blk[pb[tid][1]] && blk[pb[q][2]] == blk[pb[tid][2]]) {
-        s = q;
-        break;
-      }
-    slot_of[tid] = s;
+  constexpr int pb[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  // slot[p]: the first ordering whose cube has the same blocks as ordering p (orbits with equal
+  // blocks have fewer than six distinct cubes); only orderings with slot[p] == p are staged
+  int slot[6];
+#pragma unroll
+  for (int p = 0; p < 6; ++p) {
+    slot[p] = p;
+#pragma unroll
+    for (int q = p - 1; q >= 0; --q)
+      if (blk[pb[q][0]] == blk[pb[p][0]] && blk[pb[q][1]] == blk[pb[p][1]] && blk[pb[q][2]] == blk[pb[p][2]]) slot[p] = q;
   }
-  double acc = 0.0;
-  if (wocc != 0.0) {
-    const int64_t vv = (int64_t)v * v;
-    const int occ[3] = {i, j, k};
-    // stage the orbit's T1 vectors, (ia|jb) blocks and virtual energies
-    for (int e = tid; e < 3 * 3 * CT; e += 256) {
-      const int x = e & 7, b = (e >> 3) % 3, q = e / 24;
-      const int a = blk[b] * CT + x;
-      s_t1[q][b][x] = a < v ? t1p[a + (int64_t)v * occ[q]] : 0.0;
-    }
-    for (int e = tid; e < 3 * 9 * CT * CT; e += 256) {
-      const int xy = e & 63, pr = (e >> 6) % 9, m = e / 576;
-      const int bx = pr % 3, by = pr / 3;
-      const int a = blk[bx] * CT + (xy & 7), b = blk[by] * CT + (xy >> 3);
-      const int oa = m == 0 ? j : i, ob2 = m == 2 ? j : k;   // (jk) (ik) (ij)
-      s_vv[m][bx][by][xy] = (a < v && b < v) ? vvoo[a + (int64_t)v * b + vv * (oa + (int64_t)o * ob2)] : 0.0;
+  const int64_t vv = (int64_t)v * v;
+  const double* vv_jk = vvoo + vv * (j + (int64_t)o * k0);   // (jk) block of k0; + vv*o per k
+  const double* vv_ik = vvoo + vv * (i + (int64_t)o * k0);
+  const double* vv_ij = vvoo + vv * (i + (int64_t)o * j);
+  // element e = (3 bx + by) * 64 + xy of the staged blocks s_vv[.][bx][by][xy]
+  auto vv_elem = [&](const double* base, int e) {
+    const int xy = e & 63, pr = e >> 6, bx = pr / 3, by = pr % 3;
+    const int a = blk[bx] * CT + (xy & 7), b = blk[by] * CT + (xy >> 3);
+    return (a < v && b < v) ? base[a + (int64_t)v * b] : 0.0;
+  };
+  // k-independent blocks: T1 of i and j, (ij), virtual energies
+  if (tid < 2 * 3 * CT) {
+    const int x = tid & 7, b = (tid >> 3) % 3, q = tid / 24;
+    const int a = blk[b] * CT + x;
+    s_t1[q][b][x] = a < v ? t1p[a + (int64_t)v * (q == 0 ? i : j)] : 0.0;
+  }
+  for (int e = tid; e < 9 * CT * CT; e += 256) (&s_vv[2][0][0][0])[e] = vv_elem(vv_ij, e);
+  if (tid < 3 * CT) {
+    const int a = blk[tid >> 3] * CT + (tid & 7);
+    s_ev[tid >> 3][tid & 7] = a < v ? ev[a] : 0.0;
+  }
+  // k-dependent blocks of k = k0 + kk: T1 of k, (jk), (ik)
+  auto stage_k = [&](int kk) {
+    const int64_t ko = vv * o * kk;
+    for (int e = tid; e < 2 * 9 * CT * CT; e += 256) {
+      const int m = e >= 9 * CT * CT;
+      (&s_vv[m][0][0][0])[e - m * 9 * CT * CT] = vv_elem((m ? vv_ik : vv_jk) + ko, e - m * 9 * CT * CT);
     }
     if (tid < 3 * CT) {
       const int a = blk[tid >> 3] * CT + (tid & 7);
-      s_ev[tid >> 3][tid & 7] = a < v ? ev[a] : 0.0;
+      s_t1[2][tid >> 3][tid & 7] = a < v ? t1p[a + (int64_t)v * (k0 + kk)] : 0.0;
     }
-    __syncthreads();
+  };
+  // the thread's two elements of every cube: local (la, lb, lc), W offset within a cube, LDS
+  // offsets of the element and of its five index permutations
+  int la[2], lb[2], lc[2], pidx[2][6];
+  int64_t toff[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = tid + 256 * h;
+    la[h] = e & 7; lb[h] = (e >> 3) & 7; lc[h] = e >> 6;
+    toff[h] = la[h] + (int64_t)v * lb[h] + vv * lc[h];
+    pidx[h][0] = Cube<TV>::off(la[h], lb[h], lc[h]);
+    pidx[h][1] = Cube<TV>::off(lb[h], lc[h], la[h]);
+    pidx[h][2] = Cube<TV>::off(lc[h], la[h], lb[h]);
+    pidx[h][3] = Cube<TV>::off(la[h], lc[h], lb[h]);
+    pidx[h][4] = Cube<TV>::off(lb[h], la[h], lc[h]);
+    pidx[h][5] = Cube<TV>::off(lc[h], lb[h], la[h]);
+  }
+  int64_t cbase[6];
+  bool ok[6][2];
+#pragma unroll
+  for (int p = 0; p < 6; ++p) {
+    const int P0 = blk[pb[p][0]] * CT, Q0 = blk[pb[p][1]] * CT, R0 = blk[pb[p][2]] * CT;
+    cbase[p] = P0 + (int64_t)v * Q0 + vv * R0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) ok[p][h] = P0 + la[h] < v && Q0 + lb[h] < v && R0 + lc[h] < v;
+  }
+  TW wc[6][2], wn[6][2];
+  auto load_w = [&](int kk, TW (&r)[6][2]) {
     const TW* Wk = W + vv * v * kk;
-    double wreg[6][2];
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) r[p][h] = (slot[p] == p && ok[p][h]) ? Wk[cbase[p] + toff[h]] : (TW)0;
+  };
+  double acc = 0.0;
+  if (kb < ke) {
+    stage_k(kb);
+    load_w(kb, wc);
+  }
+  __syncthreads();
+  for (int kk = kb; kk < ke; ++kk) {
+    const int k = k0 + kk;
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
-      wreg[p][0] = wreg[p][1] = 0.0;
-      if (slot_of[p] != p) continue;
+      if (slot[p] != p) continue;
       const int pa = pb[p][0], pbb = pb[p][1], pc = pb[p][2];
-      const int P0 = blk[pa] * CT, Q0 = blk[pbb] * CT, R0 = blk[pc] * CT;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int e = tid + 256 * h;
-        const int la = e & 7, lb = (e >> 3) & 7, lc = e >> 6;
-        const int a = P0 + la, b = Q0 + lb, c = R0 + lc;
-        double w = 0.0, val = 0.0;
-        if (a < v && b < v && c < v) {
-          w = (double)Wk[a + v * b + vv * c];
-          val = w + s_t1[0][pa][la] * s_vv[0][pbb][pc][lb + 8 * lc] + s_vv[1][pa][pc][la + 8 * lc] * s_t1[1][pbb][lb] +
-                s_vv[2][pa][pbb][la + 8 * lb] * s_t1[2][pc][lc];
-        }
-        wreg[p][h] = w;
-        Vc[p][vpad(la, lb, lc)] = val;
+        const double wv = (double)wc[p][h];
+        const double val = wv + s_t1[0][pa][la[h]] * s_vv[0][pbb][pc][lb[h] + 8 * lc[h]] +
+                           s_vv[1][pa][pc][la[h] + 8 * lc[h]] * s_t1[1][pbb][lb[h]] +
+                           s_vv[2][pa][pbb][la[h] + 8 * lb[h]] * s_t1[2][pc][lc[h]];
+        Vc[p][pidx[h][0]] = ok[p][h] ? (TV)val : (TV)0;
       }
     }
     __syncthreads();
+    // the next k's blocks and W (s_vv[0..1] and s_t1[2] are not read by the energy sum)
+    if (kk + 1 < ke) {
+      stage_k(kk + 1);
+      load_w(kk + 1, wn);
+    }
+    const double wocc = 2.0 - (i == j ? 1.0 : 0.0) - (j == k ? 1.0 : 0.0);
     const double Dijk = eo[i] + eo[j] + eo[k];
+    double ak = 0.0;
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
-      if (slot_of[p] != p) continue;
+      if (slot[p] != p) continue;
       const int X = pb[p][0], Y = pb[p][1], Z = pb[p][2];
-      // slots holding the cubes (Y,Z,X) (Z,X,Y) (X,Z,Y) (Y,X,Z) (Z,Y,X)
-      auto find = [&](int x0, int y0, int z0) {
-        int r = 0;
-        for (int q = 0; q < 6; ++q)
-          if (blk[pb[q][0]] == blk[x0] && blk[pb[q][1]] == blk[y0] && blk[pb[q][2]] == blk[z0]) { r = slot_of[q]; break; }
-        return r;
-      };
-      const double* Vbca = Vc[find(Y, Z, X)];
-      const double* Vcab = Vc[find(Z, X, Y)];
-      const double* Vacb = Vc[find(X, Z, Y)];
-      const double* Vbac = Vc[find(Y, X, Z)];
-      const double* Vcba = Vc[find(Z, Y, X)];
+      const TV* Vbca = Vc[slot[ord_index(Y, Z)]];
+      const TV* Vcab = Vc[slot[ord_index(Z, X)]];
+      const TV* Vacb = Vc[slot[ord_index(X, Z)]];
+      const TV* Vbac = Vc[slot[ord_index(Y, X)]];
+      const TV* Vcba = Vc[slot[ord_index(Z, Y)]];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int e = tid + 256 * h;
-        const int la = e & 7, lb = (e >> 3) & 7, lc = e >> 6;
-        const double w = wreg[p][h];
-        if (w != 0.0) {
-          const double Zs = 4.0 * Vc[p][vpad(la, lb, lc)] + Vbca[vpad(lb, lc, la)] + Vcab[vpad(lc, la, lb)] -
-                            2.0 * (Vacb[vpad(la, lc, lb)] + Vbac[vpad(lb, la, lc)] + Vcba[vpad(lc, lb, la)]);
-          acc += w * Zs / (Dijk - s_ev[X][la] - s_ev[Y][lb] - s_ev[Z][lc]);
-        }
+        const double Zs = 4.0 * (double)Vc[p][pidx[h][0]] + (double)Vbca[pidx[h][1]] + (double)Vcab[pidx[h][2]] -
+                          2.0 * ((double)Vacb[pidx[h][3]] + (double)Vbac[pidx[h][4]] + (double)Vcba[pidx[h][5]]);
+        ak += (double)wc[p][h] * Zs * rcp_nr(Dijk - s_ev[X][la[h]] - s_ev[Y][lb[h]] - s_ev[Z][lc[h]]);
       }
     }
-    acc *= wocc;
+    acc += wocc * ak;
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) wc[p][h] = wn[p][h];
   }
   const double t = block_sum(acc);
-  if (tid == 0) partials[blockIdx.x + (int64_t)gridDim.x * kk] = t;
+  if (tid == 0) partials[blockIdx.x + (int64_t)gridDim.x * blockIdx.y] = t;
 }
 
 int triples_orbit_count(int v) {
@@ -468,9 +537,16 @@ void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, c
                     const double* t1p, const double* vvoo, const double* eo, const double* ev, double* partials,
                     int64_t* nblocks) {
   const int norb = triples_orbit_count(v);
-  *nblocks = (int64_t)norb * nk;
-  hipLaunchKernelGGL(k_triples_energy<TW>, dim3((unsigned)norb, (unsigned)nk), dim3(256), 0, s, o, v, i, j, k0, W,
-                     orbits, t1p, vvoo, eo, ev, partials);
+  // k values per workgroup: enough workgroups to fill the chip several times over, and a run of
+  // k long enough to amortise the k-independent staging and overlap W loads with the energy sum
+  // (FERMIHIP_TRIPLES_KPW=n fixes it, for tuning and for the tests that compare runs of k)
+  int kpw = 1;
+  while (kpw < 8 && (int64_t)norb * ((nk + 2 * kpw - 1) / (2 * kpw)) >= 4096) kpw *= 2;
+  if (const char* env = std::getenv("FERMIHIP_TRIPLES_KPW")) kpw = std::max(1, std::min(64, std::atoi(env)));
+  const int nky = (nk + kpw - 1) / kpw;
+  *nblocks = (int64_t)norb * nky;
+  hipLaunchKernelGGL(k_triples_energy<TW>, dim3((unsigned)norb, (unsigned)nky), dim3(256), 0, s, o, v, i, j, k0, nk,
+                     kpw, W, orbits, t1p, vvoo, eo, ev, partials);
   FH_HIP(hipGetLastError());
 }
 template void triples_energy<double>(hipStream_t, int, int, int, int, int, int, const double*, const int*, const double*,

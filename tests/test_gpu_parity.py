@@ -106,6 +106,40 @@ def test_triples_matches_oracle(o, v, Q, seed, t1s):
     assert abs(e_gpu - e_ref) < 1e-12 * abs(e_ref)
 
 
+@pytest.mark.parametrize("kpw", ["1", "3", "8"])
+def test_triples_runs_of_k(kpw, monkeypatch):
+    """Each workgroup of the (T) energy kernel walks a run of k values; forced run lengths (1, a
+    length that leaves a partial run, longer than any pair's k range) reproduce the oracle at FP64
+    and the FP32 / split-BF16x3 bounds of the W build."""
+    o, v, Q, seed = 9, 21, 40, 6
+    say(f"triples runs of k={kpw} o={o} v={v} Q={Q} seed={seed}")
+    monkeypatch.setenv("FERMIHIP_TRIPLES_KPW", kpw)
+    eo, ev, Boo, Bov, Bvv = random_df(o, v, Q, seed)
+    m = R.build_moints(Boo, Bov, Bvv)
+    T1, T2 = random_amplitudes(o, v, seed, s1=0.1)
+    c = ctx_for(eo, ev, Boo, Bov, Bvv)
+    e_ref = TT.rccsdpt_ijk(T1, T2, m, eo, ev)
+    assert abs(c.triples(T1, T2) - e_ref) < 1e-12 * abs(e_ref)
+    assert abs(c.triples(T1, T2, prec_mode=1) - e_ref) < 1e-6 * abs(e_ref)
+    assert abs(c.triples(T1, T2, prec_mode=2) - e_ref) < 3e-5 * abs(e_ref)
+
+
+def test_triples_runs_of_k_water10(monkeypatch):
+    """At the (H2O)10 shape (o=40, v=190) the launcher picks runs of several k per workgroup; the
+    energy equals the one-k-per-workgroup result up to FP64 summation order."""
+    say("triples runs of k water10 (40,190,840)")
+    from fermihip import synth
+    eo, ev, Boo, Bov, Bvv, _, _ = synth.water_cluster_like(10, 0)
+    c = ctx_for(eo, ev, Boo, Bov, Bvv)
+    T1, T2 = c.guess()
+    T1 = 0.01 * np.random.default_rng(5).standard_normal(T1.shape)
+    e_auto = c.triples(T1, T2, prec_mode=2)
+    monkeypatch.setenv("FERMIHIP_TRIPLES_KPW", "1")
+    e_one = c.triples(T1, T2, prec_mode=2)
+    say(f"  auto {e_auto:.15e} one {e_one:.15e}")
+    assert abs(e_auto - e_one) < 1e-12 * abs(e_one)
+
+
 def test_full_size_properties_water4():
     """o=16, v=76, Q=336: MP2 closed form, closed-shell symmetry of the update, (T) scaling law,
     bitwise determinism of repeated calls."""
