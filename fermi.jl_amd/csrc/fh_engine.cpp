@@ -107,6 +107,11 @@ struct Ctx {
   // 4-index blocks (Julia index order, column-major)
   Buf OOOO, OOOV, OOVV, OVOV, OVVV, Lovov, Vperm, Pt;
   Buf Vperm32;  // FP32 copy of Vperm for the mixed-precision ladder (made on first use)
+  // GEMM operands made from the integral blocks above (a permuted FP64 copy, or an FP32 copy for
+  // split-BF16x3): the same for every CC iteration, so made once and kept, up to opcache_cap
+  // bytes; key "<block>|<index order>|<d or f>"
+  std::map<std::string, Buf> opcache;
+  size_t opcache_bytes = 0, opcache_cap = 0;
   // device-resident amplitudes (last guess / solve)
   Buf T1, T2;
   bool have_amp = false;
@@ -278,46 +283,77 @@ static void contract(Ctx& c, double alpha, const TView& A, const std::string& ia
   for (char ch : N) Nsz *= LB.dim(ch);
   for (char ch : Kord) Ksz *= LA.dim(ch);
 
-  // A as logical Msz x Ksz
-  Buf tA, tB, tC;
-  const double* pA = A.p;
-  bool transA = false;
-  int64_t lda;
-  if (is_concat(ia, M, Kord)) { lda = std::max<int64_t>(Msz, 1); }
-  else if (is_concat(ia, Kord, M)) { transA = true; lda = std::max<int64_t>(Ksz, 1); }
-  else {
-    tA = dalloc(c, A.size());
-    permute_to(c, A, ia, tA.d(), M + Kord, 1.0, 0.0);
-    pA = tA.d();
-    lda = std::max<int64_t>(Msz, 1);
-  }
-  const double* pB = B.p;
-  bool transB = false;
-  int64_t ldb;
-  if (is_concat(ib, Kord, N)) { ldb = std::max<int64_t>(Ksz, 1); }
-  else if (is_concat(ib, N, Kord)) { transB = true; ldb = std::max<int64_t>(Nsz, 1); }
-  else {
-    tB = dalloc(c, B.size());
-    permute_to(c, B, ib, tB.d(), Kord + N, 1.0, 0.0);
-    pB = tB.d();
-    ldb = std::max<int64_t>(Ksz, 1);
-  }
   const int64_t wsz = gemm_splitk_elems(Msz, Nsz, Ksz, 1);
   Buf work;
   if (wsz) work = dalloc(c, wsz);
   // split-BF16x3 for the o^3 v^3-class GEMMs of the mixed phase, on FP32 copies of the operands
   // (half the operand traffic of FP64 loads); smaller GEMMs (and the split-K shapes) stay on FP32 MFMA
   const bool bf3 = c.lowp == 2 && wsz == 0 && (double)Msz * (double)Nsz * (double)Ksz >= 1e9;
-  auto gemm = [&](bool ta, bool tb, int64_t m, int64_t n, double al, const double* a, int64_t la, const double* b,
-                  int64_t lb, double be, const double* cc, double* d, int64_t ld) {
+
+  // The GEMM's copy of an operand: the tensor itself when its index order is GEMM-ready and the
+  // GEMM reads FP64, else a permuted and/or FP32 copy.  Copies of the integral blocks go to the
+  // operand cache and serve the later calls (every CC iteration makes the same ones).
+  auto operand = [&](const TView& X, const std::string& ix, bool ready, const std::string& layout, Buf& tmp,
+                     Buf& tmpf) -> const void* {
+    if (ready && !bf3) return X.p;
+    const char* blk = X.p == c.OVOV.d() ? "ovov" : X.p == c.OVVV.d() ? "ovvv" : X.p == c.OOOV.d() ? "ooov"
+                      : X.p == c.OOVV.d() ? "oovv" : X.p == c.OOOO.d() ? "oooo" : X.p == c.Lovov.d() ? "Lovov"
+                      : nullptr;
+    std::string key;
+    if (blk) {
+      key = std::string(blk) + "|" + (ready ? ix : layout) + (bf3 ? "|f" : "|d");
+      auto it = c.opcache.find(key);
+      if (it != c.opcache.end()) return it->second.p;
+    }
+    const int64_t n = X.size();
+    const double* src = X.p;
+    if (!ready) {
+      tmp = dalloc(c, n);
+      permute_to(c, X, ix, tmp.d(), layout, 1.0, 0.0);
+      src = tmp.d();
+    }
+    Buf* made = &tmp;
     if (bf3) {
-      Buf af(c.pool.get(), (size_t)(m * Ksz) * sizeof(float)), bf(c.pool.get(), (size_t)(n * Ksz) * sizeof(float));
-      cast_f64_to_f32(c.s, m * Ksz, a, af.f());
-      cast_f64_to_f32(c.s, n * Ksz, b, bf.f());
-      gemm_bf3<float, double>(c.s, ta, tb, m, n, Ksz, al, af.f(), la, 0, bf.f(), lb, 0, be, cc, ld, 0, d, ld, 0, 1, 0,
-                              nullptr);
-    } else gemm_f64(c.s, ta, tb, m, n, Ksz, al, a, la, 0, b, lb, 0, be, cc, ld, 0, d, ld, 0, 1, 0, work.d(), wsz, nullptr,
-                  c.lowp != 0);
+      tmpf = Buf(c.pool.get(), (size_t)n * sizeof(float));
+      cast_f64_to_f32(c.s, n, src, tmpf.f());
+      made = &tmpf;
+    }
+    const void* out = made->p;
+    if (blk) {
+      if (!c.opcache_cap) {
+        size_t fr = 0, tot = 0;
+        FH_HIP(hipMemGetInfo(&fr, &tot));
+        c.opcache_cap = std::max<size_t>(1, std::min<size_t>(fr / 4, (size_t)48 << 30));
+      }
+      if (c.opcache_bytes + made->bytes <= c.opcache_cap) {
+        c.opcache_bytes += made->bytes;
+        c.opcache[key] = std::move(*made);
+      }
+    }
+    return out;
+  };
+  // A as logical Msz x Ksz, B as Ksz x Nsz
+  Buf tA, tB, tAf, tBf, tC;
+  bool transA = false, readyA = true;
+  int64_t lda;
+  if (is_concat(ia, M, Kord)) { lda = std::max<int64_t>(Msz, 1); }
+  else if (is_concat(ia, Kord, M)) { transA = true; lda = std::max<int64_t>(Ksz, 1); }
+  else { readyA = false; lda = std::max<int64_t>(Msz, 1); }
+  const void* pA = operand(A, ia, readyA, M + Kord, tA, tAf);
+  bool transB = false, readyB = true;
+  int64_t ldb;
+  if (is_concat(ib, Kord, N)) { ldb = std::max<int64_t>(Ksz, 1); }
+  else if (is_concat(ib, N, Kord)) { transB = true; ldb = std::max<int64_t>(Nsz, 1); }
+  else { readyB = false; ldb = std::max<int64_t>(Ksz, 1); }
+  const void* pB = operand(B, ib, readyB, Kord + N, tB, tBf);
+  auto gemm = [&](bool ta, bool tb, int64_t m, int64_t n, double al, const void* a, int64_t la, const void* b,
+                  int64_t lb, double be, const double* cc, double* d, int64_t ld) {
+    if (bf3)
+      gemm_bf3<float, double>(c.s, ta, tb, m, n, Ksz, al, static_cast<const float*>(a), la, 0,
+                              static_cast<const float*>(b), lb, 0, be, cc, ld, 0, d, ld, 0, 1, 0, nullptr);
+    else
+      gemm_f64(c.s, ta, tb, m, n, Ksz, al, static_cast<const double*>(a), la, 0, static_cast<const double*>(b), lb, 0,
+               be, cc, ld, 0, d, ld, 0, 1, 0, work.d(), wsz, nullptr, c.lowp != 0);
   };
   if (cmode == 0) {
     gemm(transA, transB, Msz, Nsz, alpha, pA, lda, pB, ldb, beta, C.p, C.p, Msz);
@@ -375,6 +411,8 @@ static void prepare(Ctx& c) {
   c.Vperm.reset();   // the ladder operands follow on first use (prepare_ladder)
   c.Vperm32.reset();
   c.Pt.reset();
+  c.opcache.clear();  // copies of the previous blocks
+  c.opcache_bytes = 0;
   c.prepared = true;
   c.lowp = keep;
 }
@@ -1059,6 +1097,7 @@ int fh_set_df(fh_ctx* x, int o, int v, int naux, const double* eps_o, const doub
     c.have_amp = false;
     c.OOOO.reset(); c.OOOV.reset(); c.OOVV.reset(); c.OVOV.reset(); c.OVVV.reset(); c.Lovov.reset();
     c.Vperm.reset(); c.Vperm32.reset(); c.Pt.reset(); c.T1.reset(); c.T2.reset();
+    c.opcache.clear(); c.opcache_bytes = 0;
     c.have_model = false; c.pair_d.reset();
     c.o = o; c.v = v; c.naux = naux;
     const int64_t Q = naux;
