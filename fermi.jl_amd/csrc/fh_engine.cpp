@@ -626,6 +626,14 @@ static void learned_guess(Ctx& c) {
   Timer t(c, "t2_guess_learned", 0.0);
   const int h = c.mlp_h;
   const int64_t n = c.n2();
+  if (t2_mlp_fused_available(h)) {
+    Buf Y(c.pool.get(), (size_t)n * sizeof(float)), ws(c.pool.get(), t2_mlp_fused_scratch_bytes());
+    t2_mlp_fused(c.s, (int)c.o, (int)c.v, c.T2.d(), c.eo.d(), c.ev.d(), c.pair_d.d(), c.mlp_xm.f(), c.mlp_xs.f(),
+                 c.mlp_P.f(), ws.p, Y.f());
+    t2_learned_apply(c.s, (int)c.o, (int)c.v, c.mlp_ys, Y.f(), c.T2.d());
+    return;
+  }
+  // other widths: layer by layer (FP32-MFMA GEMMs), in chunks of amplitudes
   const int64_t C = std::min<int64_t>(n, (int64_t)1 << 20);
   Buf X(c.pool.get(), (size_t)(6 * C) * sizeof(float)), A(c.pool.get(), (size_t)(3 * h * C) * sizeof(float));
   Buf Y(c.pool.get(), (size_t)n * sizeof(float));

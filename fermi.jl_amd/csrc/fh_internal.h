@@ -148,6 +148,12 @@ void mlp_gather(hipStream_t s, int64_t B, int64_t off, const int64_t* perm, cons
 void t2_features(hipStream_t s, int o, int v, int64_t e0, int64_t n, const double* t2, const double* eo,
                  const double* ev, const double* dij, const float* xm, const float* xs, float* X);
 void t2_learned_apply(hipStream_t s, int o, int v, double ys, const float* Y, double* t2);
+// network output Y[o,o,v,v] for all amplitudes in one fused kernel (hidden width 128 only; wsplit:
+// t2_mlp_fused_scratch_bytes() of device memory for the bf16 weight planes)
+bool t2_mlp_fused_available(int h);
+size_t t2_mlp_fused_scratch_bytes();
+void t2_mlp_fused(hipStream_t s, int o, int v, const double* t2, const double* eo, const double* ev, const double* dij,
+                  const float* xm, const float* xs, const float* P, void* wsplit, float* Y);
 
 // workspace (doubles) that gemm_f64 would use to split K for this shape (0: no split)
 int64_t gemm_splitk_elems(int64_t M, int64_t N, int64_t K, int64_t batch);

@@ -241,4 +241,199 @@ void t2_learned_apply(hipStream_t s, int o, int v, double ys, const float* Y, do
   FH_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ fused guess inference, h = 128
+// One workgroup (256 threads = 4 wavefronts, 2 x 2 over hidden units x amplitudes) evaluates the
+// network on 128 amplitudes: features and layer 1 (K = 6) on the VALU, layers 2 and 3 as
+// [128 x 128] x [128 x 128] products on split-BF16x3 MFMA (hi/lo bf16 parts of weights and
+// activations, three products, FP32 accumulation: about 2^-16 relative per product), the output
+// layer as an in-register dot product.  Activations stay in LDS (amplitude-major rows, hi/lo
+// planes); the weights of layers 2 and 3 are split once per call into bf16 planes, row-major
+// [out][in], and read by every workgroup from L2.  tanh(x) = 1 - 2 / (e^{2x} + 1) with the
+// hardware exp2 and reciprocal (absolute error ~1e-7).
+namespace {
+constexpr int kFH = 128;        // hidden width of the fused kernel
+constexpr int kFN = 128;        // amplitudes per workgroup
+constexpr int kFRow = kFH + 8;  // LDS row of one amplitude's activations (bf16; 16-byte aligned rows)
+
+typedef __bf16 fbf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 fbf16x2 __attribute__((ext_vector_type(2)));
+typedef float ff32x16 __attribute__((ext_vector_type(16)));
+typedef float ff32x2 __attribute__((ext_vector_type(2)));
+
+// (a, b) -> packed bf16 hi parts and packed bf16 lo parts (x - hi), round to nearest even
+__device__ inline void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
+  const ff32x2 x = {a, b};
+  const fbf16x2 h = __builtin_convertvector(x, fbf16x2);
+  uint32_t hb;
+  __builtin_memcpy(&hb, &h, 4);
+  const ff32x2 r = {a - __uint_as_float(hb << 16), b - __uint_as_float(hb & 0xffff0000u)};
+  const fbf16x2 l = __builtin_convertvector(r, fbf16x2);
+  __builtin_memcpy(&lo, &l, 4);
+  hi = hb;
+}
+
+__device__ inline float fast_tanh(float x) {
+  const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);   // e^{2x} = 2^{2x log2 e}
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+}  // namespace
+
+// Wp[layer][hi/lo][out][in] from the column-major W2 / W3 of the parameter vector
+__global__ void k_mlp_split_w(const float* __restrict__ P, size_t offW2, size_t offW3, uint32_t* __restrict__ Wp) {
+  constexpr int pairs = kFH * kFH / 2;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < 2 * pairs; e += gridDim.x * 256) {
+    const int L = e / pairs, r = e - L * pairs, out = r / (kFH / 2), in = 2 * (r % (kFH / 2));
+    const float* W = P + (L ? offW3 : offW2);
+    uint32_t hi, lo;
+    split_pair(W[out + (size_t)kFH * in], W[out + (size_t)kFH * (in + 1)], hi, lo);
+    Wp[(2 * L) * pairs + r] = hi;
+    Wp[(2 * L + 1) * pairs + r] = lo;
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void k_t2_mlp_fused(int o, int v, int64_t n, const double* __restrict__ t2,
+                                                         const double* __restrict__ eo, const double* __restrict__ ev,
+                                                         const double* __restrict__ dij, const float* __restrict__ xm,
+                                                         const float* __restrict__ xs, const float* __restrict__ P,
+                                                         MlpOffsets off, const uint16_t* __restrict__ Wp,
+                                                         float* __restrict__ Y) {
+  __shared__ __attribute__((aligned(16))) uint16_t Hs[2][kFN * kFRow];  // [hi/lo][amplitude][hidden]
+  __shared__ float Xs[kFN][6];
+  __shared__ float red[2][kFN];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, wm = wid & 1, wn = wid >> 1;
+  const int64_t e0 = (int64_t)blockIdx.x * kFN;
+  if (tid < kFN) {
+    const int64_t e = e0 + tid;
+    float f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (e < n) {
+      int64_t r = e;
+      const int i = (int)(r % o); r /= o;
+      const int j = (int)(r % o); r /= o;
+      const int a = (int)(r % v);
+      const int b = (int)(r / v);
+      f[0] = (float)t2[e]; f[1] = (float)eo[i]; f[2] = (float)eo[j]; f[3] = (float)ev[a]; f[4] = (float)ev[b];
+      f[5] = dij ? (float)dij[i + (int64_t)o * j] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) Xs[tid][k] = (f[k] - xm[k]) * xs[k];
+  }
+  __syncthreads();
+  {
+    // layer 1: amplitude nn, hidden units h0 .. h0+63
+    const int nn = tid & (kFN - 1), h0 = (tid >> 7) * 64;
+    float x[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) x[k] = Xs[nn][k];
+    const float* W1 = P + off.W1;
+    const float* b1 = P + off.b1;
+#pragma unroll 4
+    for (int q = 0; q < 64; q += 2) {
+      float z[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int hh = h0 + q + t;
+        float acc = b1[hh];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc = fmaf(W1[hh + kFH * k], x[k], acc);
+        z[t] = fast_tanh(acc);
+      }
+      uint32_t hi, lo;
+      split_pair(z[0], z[1], hi, lo);
+      *reinterpret_cast<uint32_t*>(&Hs[0][nn * kFRow + h0 + q]) = hi;
+      *reinterpret_cast<uint32_t*>(&Hs[1][nn * kFRow + h0 + q]) = lo;
+    }
+  }
+  __syncthreads();
+  const int li = lane & 31, kq = 8 * (lane >> 5);
+  for (int L = 0; L < 2; ++L) {
+    // Z[hid, amp] = W[hid, :] . H[:, amp]; wavefront (wm, wn): hid 64 wm .. +63, amp 64 wn .. +63
+    ff32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][jj][r] = 0.0f;
+    const uint16_t* Wh = Wp + (size_t)(2 * L) * kFH * kFH;
+    const uint16_t* Wl = Wh + (size_t)kFH * kFH;
+#pragma unroll 2
+    for (int k0 = 0; k0 < kFH; k0 += 16) {
+      fbf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int row = wm * 64 + i * 32 + li;
+        ah[i] = *reinterpret_cast<const fbf16x8*>(Wh + row * kFH + k0 + kq);
+        al[i] = *reinterpret_cast<const fbf16x8*>(Wl + row * kFH + k0 + kq);
+      }
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj) {
+        const int col = wn * 64 + jj * 32 + li;
+        bh[jj] = *reinterpret_cast<const fbf16x8*>(&Hs[0][col * kFRow + k0 + kq]);
+        bl[jj] = *reinterpret_cast<const fbf16x8*>(&Hs[1][col * kFRow + k0 + kq]);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+          acc[i][jj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[jj], acc[i][jj], 0, 0, 0);
+          acc[i][jj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[jj], acc[i][jj], 0, 0, 0);
+          acc[i][jj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[jj], acc[i][jj], 0, 0, 0);
+        }
+    }
+    __syncthreads();  // every wavefront has read H
+    // accumulator element r of lane l: hidden row 8 (r / 4) + 4 (l / 32) + r % 4 of the block,
+    // amplitude column l % 32
+    const float* bL = P + (L ? off.b3 : off.b2);
+    if (L == 0) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+          const int col = wn * 64 + jj * 32 + li;
+#pragma unroll
+          for (int r = 0; r < 16; r += 2) {
+            const int hid = wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            uint32_t hi, lo;
+            split_pair(fast_tanh(acc[i][jj][r] + bL[hid]), fast_tanh(acc[i][jj][r + 1] + bL[hid + 1]), hi, lo);
+            *reinterpret_cast<uint32_t*>(&Hs[0][col * kFRow + hid]) = hi;
+            *reinterpret_cast<uint32_t*>(&Hs[1][col * kFRow + hid]) = lo;
+          }
+        }
+      __syncthreads();
+    } else {
+      const float* w4 = P + off.w4;
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj) {
+        float part = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int hid = wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            part = fmaf(w4[hid], fast_tanh(acc[i][jj][r] + bL[hid]), part);
+          }
+        part += __shfl_xor(part, 32, 64);
+        if (lane < 32) red[wm][wn * 64 + jj * 32 + li] = part;
+      }
+      __syncthreads();
+      if (tid < kFN && e0 + tid < n) Y[e0 + tid] = red[0][tid] + red[1][tid] + P[off.b4];
+    }
+  }
+}
+
+bool t2_mlp_fused_available(int h) { return h == kFH; }
+size_t t2_mlp_fused_scratch_bytes() { return (size_t)4 * kFH * kFH * sizeof(uint16_t); }
+
+void t2_mlp_fused(hipStream_t s, int o, int v, const double* t2, const double* eo, const double* ev, const double* dij,
+                  const float* xm, const float* xs, const float* P, void* wsplit, float* Y) {
+  const MlpOffsets off = mlp_offsets(kFH);
+  hipLaunchKernelGGL(k_mlp_split_w, dim3(64), dim3(256), 0, s, P, off.W2, off.W3, static_cast<uint32_t*>(wsplit));
+  const int64_t n = (int64_t)o * o * v * v;
+  const int64_t nb = (n + kFN - 1) / kFN;
+  FH_CHECK(nb < ((int64_t)1 << 31), -3, "t2_mlp_fused: too many amplitudes");
+  hipLaunchKernelGGL(k_t2_mlp_fused, dim3((unsigned)nb), dim3(256), 0, s, o, v, n, t2, eo, ev, dij, xm, xs, P, off,
+                     static_cast<const uint16_t*>(wsplit), Y);
+  FH_HIP(hipGetLastError());
+}
+
 }  // namespace fh

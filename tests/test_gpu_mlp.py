@@ -107,6 +107,41 @@ def test_learned_guess_assembly():
         c.guess(kind=1)
 
 
+@pytest.mark.parametrize("nw,seed", [(2, 4), (3, 5)])
+def test_learned_guess_fused_h128(nw, seed):
+    """Width-128 networks run as one fused kernel (split-BF16x3 MFMA layers, exp2-based tanh): the
+    guess matches the FP64 oracle forward pass to 3e-4 of the output scale (the bf16x3 products
+    carry ~2^-16 relative error each, tanh ~1e-7 absolute); (H2O)2 fills whole 128-amplitude
+    tiles (o^2 v^2 = 92416), (H2O)3 leaves a partial last tile (467856 = 3655 * 128 + 16)."""
+    say(f"learned guess fused h=128 (H2O){nw} seed {seed}")
+    import fermihip
+    from fermihip import synth
+    eo, ev, Boo, Bov, Bvv, so, pos = synth.water_cluster_like(nw, seed)
+    d = synth.pair_distances(so, pos)
+    c = fermihip.Context(0)
+    c.set_df(eo, ev, Boo, Bov, Bvv)
+    T1m, T2m = c.guess(kind=0)
+    h = 128
+    net = fermihip.Mlp(hidden=h)
+    p = _rand_net(h, 7 + nw, scale=0.15)
+    net.set_params(p)
+    net.x_mean = np.array([0.0, -0.7, -0.7, 1.5, 1.5, 3.0], np.float32)
+    net.x_scale = np.array([100.0, 2.0, 2.0, 0.8, 0.8, 0.3], np.float32)
+    net.y_scale = 1e-3
+    c.set_t2_model(net, d)
+    T1l, T2l = c.guess(kind=1)
+    F = M.features(T2m, eo, ev, d, net.x_mean, net.x_scale)
+    y, _ = M.forward(p.astype(np.float64), h, F.astype(np.float32).astype(np.float64))
+    o, v = T2m.shape[0], T2m.shape[2]
+    assert (o * o * v * v) % 128 == {2: 0, 3: 16}[nw]
+    Y = y.reshape(v, v, o, o).transpose(3, 2, 1, 0)
+    ref = T2m + net.y_scale * 0.5 * (Y + np.transpose(Y, (1, 0, 3, 2)))
+    err = np.max(np.abs(T2l - ref)) / (net.y_scale * max(1.0, np.max(np.abs(Y))))
+    say(f"  max error / output scale {err:.2e}")
+    assert np.array_equal(T1l, T1m)
+    assert err < 3e-4
+
+
 def test_trained_guess_gain():
     """The committed network (trained on (H2O)2-(H2O)6 clusters by train_guess.py) on a held-out
     (H2O)4 cluster: the learned guess is closer to the converged CCSD amplitudes and energy than
