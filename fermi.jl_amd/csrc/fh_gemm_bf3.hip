@@ -73,8 +73,6 @@ __device__ inline void put4(uint16_t* hi_plane, uint16_t* lo_plane, int off, con
   *reinterpret_cast<uint2*>(lo_plane + off) = make_uint2(l0, l1);
 }
 
-__device__ inline int clamp4(int64_t n) { return n <= 0 ? 0 : (n >= 4 ? 4 : (int)n); }
-
 template <typename TI, bool VEC>
 __device__ inline V4 load4(const TI* p, bool full, int n_ok) {
   V4 v;
@@ -162,46 +160,147 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
   //   per (row, k), coalesced across the lanes, and the LDS stores of consecutive lanes land on
   //   consecutive rows (a row stride of 4 rows would put every lane on the same few banks)
   // k-contiguous pattern:  k 4*(tid%KQ)..+3, rows tid/KQ + RS j, j < LB
+  // The loads are branch-free: an element outside the operand is read at a clamped, valid
+  // address and replaced by zero, and the K segment is a pointer select, so that the loads of a
+  // tile issue back to back.  A k tile entirely inside K (all but the last) skips the k tests.
   V4 ra[KG], rb[KG > LB ? KG : LB];
-  auto gload = [&](int64_t k0) {
+  // zero x unless in (a mask, not a select: the compiler keeps the load unconditional)
+  auto keep = [](float x, bool in) { return __uint_as_float(__float_as_uint(x) & (in ? 0xffffffffu : 0u)); };
+  auto ld1 = [&](const TI* p, bool in) { return keep((float)*p, in); };
+  // 4 consecutive k of row/column r of an operand, k-contiguous (p at k gk; rin: row inside)
+  auto ld4 = [&](auto ptr, int64_t r, bool rin, int64_t gk, bool kfull) {
+    V4 v;
+    if (VEC && kfull) {
+      const V4 q = load4<TI, VEC>(ptr(rin ? r : 0, gk), true, 4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v.x[e] = keep(q.x[e], rin);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const bool in = rin && gk + e < K;
+        v.x[e] = ld1(ptr(in ? r : 0, in ? gk + e : 0), in);
+      }
+    }
+    return v;
+  };
+  // Addressing hoisted out of the k loop: the thread's rows, clamped into the operand (an
+  // outside row reads a valid element and is zeroed), as offsets from the tile's first row
+  // (row-contiguous) or as column offsets (k-contiguous).  A full k tile then costs one pointer
+  // per k column (row-contiguous) or per row (k-contiguous) and an add per element.
+  constexpr int NA = TA ? KG : 4, NB = TB ? 4 : LB;
+  // row-contiguous: row index (M, N < 2^31, checked at launch); k-contiguous: ld * row, per segment
+  int arow[TA ? 1 : 4], brow[TB ? 4 : 1];
+  int64_t aoff[TA ? KG : 1], aoff2[TA && SEG ? KG : 1], boff[TB ? 1 : LB], boff2[!TB && SEG ? LB : 1];
+  bool ain[NA], bin[NB];
+  if (!TA) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t r = m0 + (tid & 31) + 32 * i;
+      ain[i] = r < M;
+      arow[i] = (int)(ain[i] ? r : M - 1);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < KG; ++j) {
+      const int64_t r = m0 + tid / KQ + RS * j;
+      ain[j] = r < M;
+      aoff[j] = lda * (ain[j] ? r : M - 1);
+      if constexpr (SEG) aoff2[j] = lda2 * (ain[j] ? r : M - 1);
+    }
+  }
+  if (TB) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t r = n0 + (tid & 31) + 32 * i;
+      bin[i] = r < N;
+      brow[i] = (int)(bin[i] ? r : N - 1);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      const int64_t r = n0 + tid / KQ + RS * j;
+      bin[j] = r < N;
+      boff[j] = ldb * (bin[j] ? r : N - 1);
+      if constexpr (SEG) boff2[j] = ldb2 * (bin[j] ? r : N - 1);
+    }
+  }
+  // a k tile inside K (K1 % 4 == 0: no 4-wide k group straddles the segments); the values of
+  // outside rows are zeroed when the tile is stored to LDS (sstore), after the MFMAs of the
+  // current tile, so that the loads stay in flight meanwhile
+  auto gload_full = [&](int64_t k0) {
     if (!TA) {
 #pragma unroll
       for (int j = 0; j < KG; ++j) {
         const int64_t gk = k0 + KG * (tid >> 5) + j;
+        const TI* cp = (!SEG || gk < K1) ? A + lda * gk : A2 + lda2 * (gk - K1);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int64_t gm = m0 + (tid & 31) + 32 * i;
-          ra[j].x[i] = (gm < M && gk < K) ? (float)*aptr(gm, gk) : 0.0f;
-        }
+        for (int i = 0; i < 4; ++i) ra[j].x[i] = (float)cp[arow[i]];
       }
     } else {
       const int64_t gk = k0 + 4 * (tid % KQ);
-      const int k_ok = clamp4(K - gk);
+      const bool s1 = !SEG || gk < K1;
 #pragma unroll
       for (int j = 0; j < KG; ++j) {
-        const int64_t gm = m0 + tid / KQ + RS * j;
-        ra[j] = load4<TI, VEC>(aptr(gm, gk), k_ok == 4 && gm < M, gm < M ? k_ok : 0);
+        ra[j] = load4<TI, VEC>(s1 ? A + aoff[j] + gk : A2 + aoff2[SEG ? j : 0] + (gk - K1), true, 4);
       }
     }
     if (TB) {
 #pragma unroll
       for (int j = 0; j < KG; ++j) {
         const int64_t gk = k0 + KG * (tid >> 5) + j;
+        const TI* rp = (!SEG || gk < K1) ? B + ldb * gk : B2 + ldb2 * (gk - K1);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int64_t gn_ = n0 + (tid & 31) + 32 * i;
-          rb[j].x[i] = (gn_ < N && gk < K) ? (float)*bptr(gn_, gk) : 0.0f;
-        }
+        for (int i = 0; i < 4; ++i) rb[j].x[i] = (float)rp[brow[i]];
       }
     } else {
       const int64_t gk = k0 + 4 * (tid % KQ);
-      const int k_ok = clamp4(K - gk);
+      const bool s1 = !SEG || gk < K1;
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        rb[j] = load4<TI, VEC>(s1 ? B + boff[j] + gk : B2 + boff2[SEG ? j : 0] + (gk - K1), true, 4);
+      }
+    }
+  };
+  // the last, partial k tile: every element checked
+  auto gload_part = [&](int64_t k0) {
+    if (!TA) {
+#pragma unroll
+      for (int j = 0; j < KG; ++j) {
+        const int64_t gk = k0 + KG * (tid >> 5) + j;
+        const bool kin = gk < K;
+        const TI* colp = aptr(0, kin ? gk : 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ra[j].x[i] = ld1(colp + arow[i], kin && ain[i]);
+      }
+    } else {
+      const int64_t gk = k0 + 4 * (tid % KQ);
+#pragma unroll
+      for (int j = 0; j < KG; ++j) {
+        const int64_t gm = m0 + tid / KQ + RS * j;
+        ra[j] = ld4(aptr, gm, gm < M, gk, false);
+      }
+    }
+    if (TB) {
+#pragma unroll
+      for (int j = 0; j < KG; ++j) {
+        const int64_t gk = k0 + KG * (tid >> 5) + j;
+        const bool kin = gk < K;
+        const TI* rowp = bptr(0, kin ? gk : 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) rb[j].x[i] = ld1(rowp + brow[i], kin && bin[i]);
+      }
+    } else {
+      const int64_t gk = k0 + 4 * (tid % KQ);
 #pragma unroll
       for (int j = 0; j < LB; ++j) {
         const int64_t gcol = n0 + tid / KQ + RS * j;
-        rb[j] = load4<TI, VEC>(bptr(gcol, gk), k_ok == 4 && gcol < N, gcol < N ? k_ok : 0);
+        rb[j] = ld4(bptr, gcol, gcol < N, gk, false);
       }
     }
+  };
+  auto gload = [&](int64_t k0) {
+    if (k0 + BK <= K) gload_full(k0);
+    else gload_part(k0);
   };
   // r[j].x[i] = X[row (tid&31)+32i][k KG(tid>>5)+j]: gather the KG k of each row
   auto put_rows = [&](uint16_t* hi, uint16_t* lo, const V4* r) {
@@ -222,6 +321,29 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
       if (j < nl) put4(hi, lo, (tid / KQ + RS * j) * kRow + 4 * (tid % KQ), r[j]);
   };
   auto sstore = [&](int buf) {
+    // zero the values of rows outside the operand (loaded from a clamped address)
+    if (!TA) {
+#pragma unroll
+      for (int j = 0; j < KG; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ra[j].x[i] = keep(ra[j].x[i], ain[i]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < KG; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ra[j].x[e] = keep(ra[j].x[e], ain[j]);
+    }
+    if (TB) {
+#pragma unroll
+      for (int j = 0; j < KG; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) rb[j].x[i] = keep(rb[j].x[i], bin[i]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < LB; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rb[j].x[e] = keep(rb[j].x[e], bin[j]);
+    }
     if (!TA) put_rows(As[buf][0], As[buf][1], ra);
     else put_k(As[buf][0], As[buf][1], ra, KG);
     if constexpr (TB) put_rows(Bs[buf][0], Bs[buf][1], rb);
@@ -400,6 +522,7 @@ static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double al
   const int bn = (TAG == 2 && N <= 256) ? 64 : 128;
   const int64_t tm = (M + kBM - 1) / kBM, tn = (N + bn - 1) / bn;
   FH_CHECK(tm * tn < (int64_t)1 << 31 && batch < 65536, -3, "gemm grid too large");
+  FH_CHECK(M < ((int64_t)1 << 31) && N < ((int64_t)1 << 31), -3, "gemm: M and N must be below 2^31");
   dim3 grid((unsigned)(tm * tn), (unsigned)batch);
   int gn = 0;
   if (M > N) {
