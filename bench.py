@@ -27,7 +27,7 @@ PREC_NAMES = {"double": "FP64", "mixed": "FP32 MFMA + FP64 polish",
 
 # HBM bytes per launch of the roofline kernels, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
 # scaled by a plain 1 GiB copy (FETCH x 2.0, WRITE x 1.0): profiles/triples_bf16x3_water10_hbm_counters.md
-MEASURED_TRAFFIC = {("water10", "triples_gemm_bf16x3"): 1.622e9}
+MEASURED_TRAFFIC = {("water10", "triples_gemm_bf16x3"): 1.638e9}
 
 
 def log(*a):
@@ -175,8 +175,9 @@ def main():
         eo, ev, Boo, Bov, Bvv, sites_o, pos = work
     o, v, Q = Bov.shape[1], Bov.shape[2], Bov.shape[0]
     log(f"bench: config {args.config} ((H2O){nw}) o={o} v={v} naux={Q} seed={args.seed}")
-    # the (T)-only tensors take the MP2 amplitudes as T2
-    model = train_guess.load_model() if args.guess != "mp2" and args.config != "triples64" else None
+    # the (T)-only step (and the triples64 tensors) takes the MP2 amplitudes as T2
+    model = (train_guess.load_model() if args.guess != "mp2" and args.config != "triples64" and args.phase != "triples"
+             else None)
     if args.guess == "learned" and model is None:
         raise SystemExit("bench.py: --guess learned but no model file (fermi.jl_amd/fermihip/t2_guess_water.npz)")
     guess_kind = 1 if model is not None else 0
