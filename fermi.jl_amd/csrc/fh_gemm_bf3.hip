@@ -452,46 +452,65 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
     return row < M && col < N;
   };
   if constexpr (TAG == 2) {
-    // the (T) W build accumulates in place (C == D): per 32x32 block, load 8 of the thread's
-    // values before storing any of them
+    // the (T) W build accumulates in place (C == D): per 32x32 block, the thread's 16 values of C
+    // are loaded (unconditionally, an element outside D reading C[0]) before any is stored
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
         const Blk b = block(i, j);
+        if (beta != 0.0) {
+          int64_t off[16];
+          bool in[16];
+          float cv[16];
 #pragma unroll
-        for (int h = 0; h < 16; h += 8) {
-          float cv[8];
-          int64_t off[8];
-          bool in[8];
+          for (int e = 0; e < 16; ++e) in[e] = elem(b, e, off[e]);
 #pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            in[e] = elem(b, h + e, off[e]);
-            cv[e] = (beta != 0.0 && in[e]) ? (float)C[off[e]] : 0.0f;
+          for (int e = 0; e < 16; ++e) cv[e] = (float)C[in[e] ? off[e] : 0];
+#pragma unroll
+          for (int e = 0; e < 16; ++e)
+            if (in[e]) D[off[e]] = (TO)(alpha * (double)acc[i][j][e] + beta * (double)cv[e]);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            int64_t off;
+            if (elem(b, e, off)) D[off] = (TO)(alpha * (double)acc[i][j][e]);
           }
-#pragma unroll
-          for (int e = 0; e < 8; ++e)
-            if (in[e]) D[off[e]] = (TO)(alpha * (double)acc[i][j][h + e] + beta * (double)cv[e]);
         }
       }
   } else {
+    // FP64 output: with beta != 0, per half block the thread's 8 values of C are loaded
+    // (unconditionally, an element outside D reading C[0]) before the 8 stores
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
         const Blk b = block(i, j);
+        if (beta != 0.0) {
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-          int64_t off;
-          if (elem(b, e, off)) {
-            double val = alpha * (double)acc[i][j][e];
-            if (beta != 0.0) {
-              const int li = lane & 31, ri = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+          for (int h = 0; h < 16; h += 8) {
+            int64_t off[8], coff[8];
+            bool in[8];
+            TO cv[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              in[e] = elem(b, h + e, off[e]);
+              const int li = lane & 31, ri = 8 * ((h + e) >> 2) + 4 * (lane >> 5) + ((h + e) & 3);
               const int64_t row = m0 + wm * 64 + i * 32 + (LM ? li : ri),
                             col = n0 + wn * (BN / 2) + j * 32 + (LM ? ri : li);
-              val += beta * (double)(mapped ? C[off] : C[row + ldc * col]);
+              coff[e] = in[e] ? (mapped ? off[e] : row + ldc * col) : 0;
             }
-            D[off] = (TO)val;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) cv[e] = C[coff[e]];
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              if (in[e]) D[off[e]] = (TO)(alpha * (double)acc[i][j][h + e] + beta * (double)cv[e]);
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            int64_t off;
+            if (elem(b, e, off)) D[off] = (TO)(alpha * (double)acc[i][j][e]);
           }
         }
       }
