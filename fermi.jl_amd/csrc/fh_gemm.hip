@@ -173,29 +173,74 @@ __global__ __launch_bounds__(256, OCC) void k_gemm(
   // row r -> (r % rmod)*rs0 + (r / rmod)*rs1, column c -> (c % cmod)*cs0 + (c / cmod)*cs1
   // (cmod = 0: c*cs0), which lets a GEMM accumulate into a permuted tensor.  LM is chosen so
   // that the lanes of one store run along the unit-stride axis of the output.
+  // The map's div/mod is done once per row and per column the thread owns (rows depend on (i,
+  // lane) and columns on (j, register) when LM, the other way round otherwise); with beta != 0
+  // every C value of the thread is loaded (unconditionally, an element outside D reading C[0])
+  // before the first store.
+  const int li = lane & 15;
+  auto rix = [&](int rr) { return F32 ? 4 * (lane >> 4) + rr : (lane >> 4) + 4 * rr; };
+  auto map_row = [&](int64_t row) { return om.rmod > 0 ? (row % om.rmod) * om.rs0 + (row / om.rmod) * om.rs1 : row; };
+  auto map_col = [&](int64_t col, int64_t ld) {
+    return om.rmod > 0 ? (om.cmod > 0 ? (col % om.cmod) * om.cs0 + (col / om.cmod) * om.cs1 : col * om.cs0) : ld * col;
+  };
+  constexpr int RR = LM ? 1 : 4, CR = LM ? 4 : 1;
+  int64_t roff[MB][RR], coff[NB][CR], ccoff[NB][CR];
+  bool rok[MB][RR], cok[NB][CR];
 #pragma unroll
   for (int i = 0; i < MB; ++i)
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
+    for (int q = 0; q < RR; ++q) {
+      const int64_t row = m0 + wm * WM + i * 16 + (LM ? li : rix(q));
+      rok[i][q] = row < M;
+      roff[i][q] = map_row(rok[i][q] ? row : 0);
+    }
 #pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int li = lane & 15, ri = F32 ? 4 * (lane >> 4) + rr : (lane >> 4) + 4 * rr;
-        const int64_t row = m0 + wm * WM + i * 16 + (LM ? li : ri);
-        const int64_t col = n0 + wn * WN + j * 16 + (LM ? ri : li);
-        if (row < M && col < N) {
-          int64_t off;
-          if (om.rmod > 0) {
-            const int64_t roff = (row % om.rmod) * om.rs0 + (row / om.rmod) * om.rs1;
-            const int64_t coff = om.cmod > 0 ? (col % om.cmod) * om.cs0 + (col / om.cmod) * om.cs1 : col * om.cs0;
-            off = roff + coff;
-          } else {
-            off = row + ldd * col;
-          }
-          double val = alpha * (double)acc[i][j][rr];
-          if (beta != 0.0) val += beta * (double)(om.rmod > 0 ? C[off] : C[row + ldc * col]);
-          D[off] = (TO)val;
+  for (int j = 0; j < NB; ++j)
+#pragma unroll
+    for (int q = 0; q < CR; ++q) {
+      const int64_t col = n0 + wn * WN + j * 16 + (LM ? rix(q) : li);
+      cok[j][q] = col < N;
+      coff[j][q] = map_col(cok[j][q] ? col : 0, ldd);
+      ccoff[j][q] = map_col(cok[j][q] ? col : 0, ldc);
+    }
+  auto at = [&](int i, int j, int rr, int64_t& off, int64_t& offc) {
+    const int qr = LM ? 0 : rr, qc = LM ? rr : 0;
+    off = roff[i][qr] + coff[j][qc];
+    offc = om.rmod > 0 ? off : roff[i][qr] + ccoff[j][qc];
+    return rok[i][qr] && cok[j][qc];
+  };
+  if (beta != 0.0) {
+    TO cv[MB][NB][4];
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          int64_t off, offc;
+          const bool in = at(i, j, rr, off, offc);
+          cv[i][j][rr] = C[in ? offc : 0];
         }
-      }
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          int64_t off, offc;
+          if (at(i, j, rr, off, offc)) D[off] = (TO)(alpha * (double)acc[i][j][rr] + beta * (double)cv[i][j][rr]);
+        }
+  } else {
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          int64_t off, offc;
+          if (at(i, j, rr, off, offc)) D[off] = (TO)(alpha * (double)acc[i][j][rr]);
+        }
+  }
 }
 
 // lanes along M unless only the output's column axis has unit stride
