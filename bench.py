@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--guess", default="auto", choices=["auto", "learned", "mp2"],
                     help="initial T2: learned (the committed network; auto = learned when the model file exists) "
                          "or mp2")
+    ap.add_argument("--t1-guess", default="zero", choices=["zero", "first-order"],
+                    help="initial T1: zero (the reference's) or the first-order T1 at the guess's T2")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--diis-prec", default="single", choices=["single", "double"])
     ap.add_argument("--precision", default="bf16x3", choices=["double", "mixed", "bf16x3"],
@@ -180,7 +182,7 @@ def main():
              else None)
     if args.guess == "learned" and model is None:
         raise SystemExit("bench.py: --guess learned but no model file (fermi.jl_amd/fermihip/t2_guess_water.npz)")
-    guess_kind = 1 if model is not None else 0
+    guess_kind = (1 if model is not None else 0) + (2 if args.t1_guess == "first-order" else 0)
     pair_d = synth.pair_distances(sites_o, pos)
     ctx = fermihip.Context(0)
     b3e, b3r = (float(t) for t in args.bf3_switch.split(","))
@@ -193,7 +195,7 @@ def main():
     def step():
         t0 = time.perf_counter()
         ctx.set_df(eo, ev, Boo, Bov, Bvv)      # host -> HBM (untimed part of the step)
-        if guess_kind == 1:
+        if guess_kind & 1:
             ctx.set_t2_model(model, pair_d)    # network + pair distances (untimed, like the upload)
         t1 = time.perf_counter()
         if args.phase != "triples":
@@ -311,9 +313,11 @@ def main():
         "dtype": {"double": "f64", "mixed": "f32/f64", "bf16x3": "bf16x3/f32/f64"}[args.precision], "data": "synthetic (seeded DF tensors of the cluster's shape)",
         "config": {"workload": f"DF-RCCSD(T) {PREC_NAMES[args.precision]}, {args.config} shape: o={o} v={v} naux={Q} (synthetic "
                                f"(H2O){nw} lattice-cluster DF tensors, seed {args.seed}), "
-                               f"{'learned' if guess_kind else 'MP2'} T2 guess",
+                               f"{'learned' if guess_kind & 1 else 'MP2'} T2 guess"
+                               f"{', first-order T1' if guess_kind & 2 else ''}",
                    "o": o, "v": v, "naux": Q, "precision": args.precision, "diis_prec": args.diis_prec,
-                   "polish_iters": args.polish_iters, "guess": "learned" if guess_kind else "mp2",
+                   "polish_iters": args.polish_iters, "guess": "learned" if guess_kind & 1 else "mp2",
+                   "t1_guess": args.t1_guess,
                    "bf3_switch": args.bf3_switch, "mixed_switch": args.mixed_switch,
                    "parallelism": "single GPU"},
         "energies": {"ccsd_correlation": r.correlation, "triples": Et, "niter": r.niter, "converged": bool(r.converged),

@@ -437,8 +437,9 @@ static void prepare_ladder(Ctx& c) {
 // ------------------------------------------------------------------ CCSD numerators
 // R1 = numerator of the T1 update (RCCSDHelper.jl:57-85); R2 = numerator of the T2 update
 // (RCCSDHelper.jl:123-220), both without the diagonal Fock term.
-static void residual(Ctx& c, const double* t1p, const double* t2p, double* R1, double* R2) {
-  prepare_ladder(c);
+// t1_only: the T1 numerator alone (R2 is not touched)
+static void residual(Ctx& c, const double* t1p, const double* t2p, double* R1, double* R2, bool t1_only = false) {
+  if (!t1_only) prepare_ladder(c);
   const int64_t o = c.o, v = c.v;
   const double o_ = (double)o, v_ = (double)v;
   Timer tm(c, "residual", 0.0);
@@ -483,6 +484,7 @@ static void residual(Ctx& c, const double* t1p, const double* t2p, double* R1, d
   contract(c, -1.0, oovv, "kiac", t1, "kc", 1.0, r1, "ia");
   contract(c, 1.0, ovvv, "kdac", u, "ikcd", 1.0, r1, "ia");
   contract(c, -1.0, ooov, "kilc", u, "klac", 1.0, r1, "ia");
+  if (t1_only) return;
 
   // ---- T2 numerator: P, then R2 = P + P[j,i,b,a]
   Buf bP = dalloc(c, c.n2());
@@ -620,6 +622,23 @@ static void mp2_guess(Ctx& c) {
 }
 
 // learned guess: MP2 amplitudes plus the network's symmetrised correction, evaluated in chunks
+// T1 = R1(T1 = 0, T2) / D1 at the guess's T2: the first-order T1 (the T1 that the first CC
+// iteration from T1 = 0 would produce), so that the iterations start with both amplitude sets
+// close to the solution
+static void first_order_t1(Ctx& c) {
+  Timer t(c, "t1_first_order", 0.0);
+  const int keep = c.lowp;
+  c.lowp = 0;
+  Buf r1 = dalloc(c, c.nov()), zero = dalloc(c, c.nov());
+  fill_f64(c.s, c.nov(), 0.0, zero.d());
+  residual(c, zero.d(), c.T2.d(), r1.d(), nullptr, true);
+  if (!c.partials.p) c.partials = dalloc(c, 4096);
+  int64_t nb = 0;
+  jacobi_update(c.s, c.o, c.v, false, c.eo.d(), c.ev.d(), r1.d(), zero.d(), nullptr, false, c.partials.d(), &nb);
+  d2d(c, c.T1.d(), r1.d(), c.nov());
+  c.lowp = keep;
+}
+
 static void learned_guess(Ctx& c) {
   FH_CHECK(c.have_model, -6, "no T2-guess model: call fh_set_t2_model after fh_set_df");
   mp2_guess(c);
@@ -1142,12 +1161,14 @@ static void upload_amp(Ctx& c, const double* T1, const double* T2) {
 int fh_t2_guess(fh_ctx* x, int kind, double* T1, double* T2) {
   return guarded([&] {
     FH_CHECK(x != nullptr, -1, "null context");
-    FH_CHECK(kind == 0 || kind == 1, -1, "fh_t2_guess: kind must be 0 (MP2) or 1 (learned)");
+    FH_CHECK(kind >= 0 && kind <= 3, -1,
+             "fh_t2_guess: kind must be 0 (MP2) or 1 (learned), plus 2 for the first-order T1");
     DeviceGuard g(x->c.device);
     Ctx& c = x->c;
     require_df(c);
-    if (kind == 1) learned_guess(c);
+    if (kind & 1) learned_guess(c);
     else mp2_guess(c);
+    if (kind & 2) first_order_t1(c);
     if (T1) d2h(c, T1, c.T1.d(), c.nov());
     if (T2) d2h(c, T2, c.T2.d(), c.n2());
     FH_HIP(hipStreamSynchronize(c.s));

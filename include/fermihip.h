@@ -93,6 +93,9 @@ int fh_prepare(fh_ctx* ctx);
  * predicted by the network attached with fh_set_t2_model, symmetrised in (ij)(ab); T1 = 0.
  * The CC iterations converge to the same amplitudes from either guess; the MP2 guess stays the
  * fallback (kind 1 fails with -6 when no network is attached).
+ * kind + 2 (2: MP2, 3: learned): T1 = R1(T1 = 0, T2)/D1 instead of 0, the first-order T1 at the
+ * guess's T2 (the T1 numerator of RCCSDHelper.jl:57-85 at T1 = 0, one FP64 evaluation; no
+ * counterpart in the reference, which starts from T1 = Fia/D1 = 0, RCCSDa.jl:58-63).
  * T1[o,v], T2[o,o,v,v] are host outputs (either may be NULL); the guess is also kept on
  * the device as the starting point of fh_ccsd_solve with NULL amplitudes. */
 int fh_t2_guess(fh_ctx* ctx, int kind, double* T1, double* T2);

@@ -140,6 +140,36 @@ def test_triples_runs_of_k_water10(monkeypatch):
     assert abs(e_auto - e_one) < 1e-12 * abs(e_one)
 
 
+@pytest.mark.parametrize("o,v,Q,seed", [(3, 7, 11, 2), (8, 30, 50, 4)])
+def test_first_order_t1_guess(o, v, Q, seed):
+    """guess(kind=2): the MP2 T2 with T1 = R1(T1 = 0, T2)/D1, the new T1 of one oracle update from
+    T1 = 0 (1e-12 relative)."""
+    say(f"first-order T1 guess o={o} v={v} Q={Q} seed={seed}")
+    eo, ev, Boo, Bov, Bvv = random_df(o, v, Q, seed)
+    m = R.build_moints(Boo, Bov, Bvv)
+    c = ctx_for(eo, ev, Boo, Bov, Bvv)
+    T1m, T2m = c.guess(kind=0)
+    T1f, T2f = c.guess(kind=2)
+    assert np.array_equal(T2f, T2m) and not np.any(T1m)
+    ref1, _ = R.update_amp(np.zeros((o, v)), T2m, m, eo, ev)
+    assert relerr(T1f, ref1) < 1e-12
+
+
+def test_first_order_t1_guess_fixed_point():
+    """From the first-order T1 (MP2 T2) the iterations reach the T1 = 0 run's energy (1e-9 Eh)."""
+    say("first-order T1 guess fixed point (water1 shape)")
+    import fermihip
+    from fermihip import synth
+    o, v, Q = synth.SHAPES["water1"]
+    eo, ev, Boo, Bov, Bvv = synth.molecular_like(o, v, Q, 0)
+    c = ctx_for(eo, ev, Boo, Bov, Bvv)
+    T1m, T2m = c.guess(kind=0)
+    T1f, T2f = c.guess(kind=2)
+    r0, _, _ = c.solve(fermihip.Options(), T1m, T2m)
+    r1, _, _ = c.solve(fermihip.Options(), T1f, T2f)
+    assert r0.converged and r1.converged and abs(r0.correlation - r1.correlation) < 1e-9
+
+
 def test_full_size_properties_water4():
     """o=16, v=76, Q=336: MP2 closed form, closed-shell symmetry of the update, (T) scaling law,
     bitwise determinism of repeated calls."""
