@@ -950,7 +950,7 @@ static double triples_t(Ctx& c, bool bf3) {
   T* AVs = AV + avsz;
   T* BVq = static_cast<T*>(BVqp.p);
   T* BVp = BVq + bvsz;
-  Buf T1p = dalloc(c, v * o), vvoo = dalloc(c, vv * o * o);
+  Buf T1p = dalloc(c, v * o), vvoo = dalloc(c, vv * o * o), vvb;
   {
     auto g = [](int64_t n) { int64_t b = (n + 255) / 256; return (unsigned)std::min<int64_t>(b, 65536 * 4); };
     hipLaunchKernelGGL(k_build_av<T>, dim3(g(avsz)), dim3(256), 0, c.s, (int)o, (int)v, (int)nd, false, c.OVVV.d(),
@@ -964,6 +964,9 @@ static double triples_t(Ctx& c, bool bf3) {
     FH_HIP(hipGetLastError());
     perm(c, tv(c.T1.d(), {o, v}), "ia", tv(T1p.d(), {v, o}), "ai", 1.0, 0.0);
     perm(c, tv(c.OVOV.d(), {o, v, o, v}), "iajb", tv(vvoo.d(), {v, v, o, o}), "abij", 1.0, 0.0);
+    vvb = dalloc(c, triples_vv_blocked_elems((int)o, (int)v));
+    triples_block_vv(c.s, (int)o, (int)v, vvoo.d(), vvb.d());
+    vvoo.reset();
   }
   // k-chunk: W for all k of a chunk, bounded to ~16 GB
   const int64_t v3 = vv * v;
@@ -1013,7 +1016,7 @@ static double triples_t(Ctx& c, bool bf3) {
         Timer te(c, "triples_energy" + sfx, 0.0);
         int64_t nb = 0;
         triples_energy(c.s, (int)o, (int)v, (int)i, (int)j, (int)k0, (int)nk, W, static_cast<const int*>(orbits.p),
-                       T1p.d(), vvoo.d(), c.eo.d(), c.ev.d(), part.d(), &nb);
+                       T1p.d(), vvb.d(), c.eo.d(), c.ev.d(), part.d(), &nb);
         hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(256), 0, c.s, nb, part.d(), acc.d());
         FH_HIP(hipGetLastError());
       }

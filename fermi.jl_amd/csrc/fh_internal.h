@@ -189,9 +189,13 @@ void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, 
 // orbits: device list of cube-block triples (A>=B>=C), 3 ints each (triples_orbits)
 int triples_orbit_count(int v);
 void triples_orbits(int v, std::vector<int>& out);
+// vvb: the (ia|jb) blocks in 8x8 tiles, made by triples_block_vv from vvoo[a,b,i,j]
+// (triples_vv_blocked_elems doubles)
+int64_t triples_vv_blocked_elems(int o, int v);
+void triples_block_vv(hipStream_t s, int o, int v, const double* vvoo, double* vvb);
 template <typename TW>  // W stored as double (FP64 path) or float (FP32 W build)
 void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TW* W, const int* orbits,
-                    const double* t1p, const double* vvoo, const double* eo, const double* ev, double* partials,
+                    const double* t1p, const double* vvb, const double* eo, const double* ev, double* partials,
                     int64_t* nblocks);
 
 }  // namespace fh

@@ -369,7 +369,7 @@ __device__ __forceinline__ double rcp_nr(double d) {
 template <typename TW>
 __global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int j, int k0, int nk, int kpw,
                                                         const TW* __restrict__ W, const int* __restrict__ orbits,
-                                                        const double* __restrict__ t1p, const double* __restrict__ vvoo,
+                                                        const double* __restrict__ t1p, const double* __restrict__ vvb,
                                                         const double* __restrict__ eo, const double* __restrict__ ev,
                                                         double* __restrict__ partials) {
   using TV = TW;
@@ -397,14 +397,16 @@ __global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int
       if (blk[pb[q][0]] == blk[pb[p][0]] && blk[pb[q][1]] == blk[pb[p][1]] && blk[pb[q][2]] == blk[pb[p][2]]) slot[p] = q;
   }
   const int64_t vv = (int64_t)v * v;
-  const double* vv_jk = vvoo + vv * (j + (int64_t)o * k0);   // (jk) block of k0; + vv*o per k
-  const double* vv_ik = vvoo + vv * (i + (int64_t)o * k0);
-  const double* vv_ij = vvoo + vv * (i + (int64_t)o * j);
+  // (ia|jb) of occupied pair (p, q) in 8x8 blocks: vvb + PS (p + o q) + 64 (A + nb B) + x + 8 y
+  const int nb = (v + CT - 1) / CT;
+  const int64_t PS = (int64_t)nb * nb * CT * CT;
+  const double* vv_jk = vvb + PS * (j + (int64_t)o * k0);   // (jk) block of k0; + PS*o per k
+  const double* vv_ik = vvb + PS * (i + (int64_t)o * k0);
+  const double* vv_ij = vvb + PS * (i + (int64_t)o * j);
   // element e = (3 bx + by) * 64 + xy of the staged blocks s_vv[.][bx][by][xy]
   auto vv_elem = [&](const double* base, int e) {
     const int xy = e & 63, pr = e >> 6, bx = pr / 3, by = pr % 3;
-    const int a = blk[bx] * CT + (xy & 7), b = blk[by] * CT + (xy >> 3);
-    return (a < v && b < v) ? base[a + (int64_t)v * b] : 0.0;
+    return base[((int64_t)(blk[bx] + nb * blk[by]) << 6) + xy];
   };
   // k-independent blocks: T1 of i and j, (ij), virtual energies
   if (tid < 2 * 3 * CT) {
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(256) void k_triples_energy(int o, int v, int i, int
   }
   // k-dependent blocks of k = k0 + kk: T1 of k, (jk), (ik)
   auto stage_k = [&](int kk) {
-    const int64_t ko = vv * o * kk;
+    const int64_t ko = PS * o * kk;
     for (int e = tid; e < 2 * 9 * CT * CT; e += 256) {
       const int m = e >= 9 * CT * CT;
       (&s_vv[m][0][0][0])[e - m * 9 * CT * CT] = vv_elem((m ? vv_ik : vv_jk) + ko, e - m * 9 * CT * CT);
@@ -532,9 +534,37 @@ void triples_orbits(int v, std::vector<int>& out) {
       for (int c = 0; c <= b; ++c) { out.push_back(a); out.push_back(b); out.push_back(c); }
 }
 
+// (ia|jb) = vvoo[a + v b + v^2 (i + o j)] -> vvb in 8x8 blocks (zero-padded), so that a staged
+// block is one contiguous 512-byte run
+__global__ void k_block_vv(int o, int v, int nb, const double* __restrict__ vvoo, double* __restrict__ vvb) {
+  const int64_t PS = (int64_t)nb * nb * CT * CT, n = PS * o * o;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const int64_t pq = e / PS;
+    const int64_t r = e - pq * PS;
+    const int xy = (int)(r & 63);
+    const int64_t blkid = r >> 6;
+    const int A = (int)(blkid % nb), B = (int)(blkid / nb);
+    const int a = A * CT + (xy & 7), b = B * CT + (xy >> 3);
+    vvb[e] = (a < v && b < v) ? vvoo[a + (int64_t)v * b + (int64_t)v * v * pq] : 0.0;
+  }
+}
+
+int64_t triples_vv_blocked_elems(int o, int v) {
+  const int64_t nb = (v + CT - 1) / CT;
+  return nb * nb * CT * CT * o * o;
+}
+
+void triples_block_vv(hipStream_t s, int o, int v, const double* vvoo, double* vvb) {
+  const int nb = (v + CT - 1) / CT;
+  const int64_t n = triples_vv_blocked_elems(o, v);
+  const int64_t g = std::min<int64_t>((n + 255) / 256, 65536 * 4);
+  hipLaunchKernelGGL(k_block_vv, dim3((unsigned)g), dim3(256), 0, s, o, v, nb, vvoo, vvb);
+  FH_HIP(hipGetLastError());
+}
+
 template <typename TW>
 void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, const TW* W, const int* orbits,
-                    const double* t1p, const double* vvoo, const double* eo, const double* ev, double* partials,
+                    const double* t1p, const double* vvb, const double* eo, const double* ev, double* partials,
                     int64_t* nblocks) {
   const int norb = triples_orbit_count(v);
   // k values per workgroup: enough workgroups to fill the chip several times over, and a run of
@@ -546,7 +576,7 @@ void triples_energy(hipStream_t s, int o, int v, int i, int j, int k0, int nk, c
   const int nky = (nk + kpw - 1) / kpw;
   *nblocks = (int64_t)norb * nky;
   hipLaunchKernelGGL(k_triples_energy<TW>, dim3((unsigned)norb, (unsigned)nky), dim3(256), 0, s, o, v, i, j, k0, nk,
-                     kpw, W, orbits, t1p, vvoo, eo, ev, partials);
+                     kpw, W, orbits, t1p, vvb, eo, ev, partials);
   FH_HIP(hipGetLastError());
 }
 template void triples_energy<double>(hipStream_t, int, int, int, int, int, int, const double*, const int*, const double*,
