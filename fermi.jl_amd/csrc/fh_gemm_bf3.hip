@@ -9,16 +9,18 @@
 // for higher-precision computations", ARITH 2019).  Three v_mfma_f32_32x32x16_bf16 issue in the
 // time of 3/16 of the FP32 MFMA work of the same tile.
 //
-// Workgroup: 256 threads = 4 wavefronts (2x2), 128x128 output tile, K staged 32 at a time
-// through double-buffered LDS (hi and lo planes of both operands, 80 KB: two workgroups per CU).
-// Each wavefront owns 64x64 = 2x2 blocks of 32x32 accumulators.  The MFMA is issued with the B
-// tile as its first operand, so the accumulator's lane index runs along the rows of D and the
-// epilogue stores are coalesced along the column-major output.
+// Workgroup: 256 threads = 4 wavefronts (2x2), 128 x BN output tile (BN = 128, or 64 for the (T)
+// launches whose N is one virtual index), K staged BK = 32 (two workgroups per CU) or 16 (three
+// per CU, short K) at a time through double-buffered LDS (hi and lo planes of both operands).
+// Each wavefront owns 64 x BN/2 as 32x32 accumulator blocks.  With LM the MFMA is issued with the
+// B tile as its first operand, so the accumulator's lane index runs along the rows of D; LM is
+// chosen so that the epilogue's stores run along the output's unit-stride axis.
 //
 // Global loads: k-contiguous operands load four consecutive k per thread (dwordx4 for FP32) when
 // the leading dimensions, batch strides and base addresses allow it (VEC), scalar loads
-// otherwise; row-contiguous operands load one element per (row, k), coalesced across lanes.
-// Tile order: the XCD-aware remap and the L2-grouped raster of the FP64/FP32 kernel.
+// otherwise; row-contiguous operands load one element per (row, k), coalesced across lanes.  The
+// loads are branch-free (clamped rows, masks applied at the LDS store).  Tile order: the
+// XCD-aware remap and the L2-grouped raster of the FP64/FP32 kernel.
 #include "fh_internal.h"
 
 #include <algorithm>
@@ -517,15 +519,10 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
   }
 }
 
-// L2 budget of the B blocks of one raster group (bytes)
-// K tile of the (T) W-build launches
-#ifndef FH_BF3_TBK
-#define FH_BF3_TBK 16
-#endif
-#ifndef FH_BF3_GROUP_BYTES
-#define FH_BF3_GROUP_BYTES (1 << 20)
-#endif
-static constexpr int64_t kGroupBytesBf3 = FH_BF3_GROUP_BYTES;
+// L2 budget of the B blocks of one raster group (bytes; 1, 4 and 16 MB measured alike)
+static constexpr int64_t kGroupBytesBf3 = 1 << 20;
+// K tile of the (T) W-build launches (K = 2 (v + o) is short: BK = 32 measured 25% slower)
+static constexpr int kTBK = 16;
 
 template <typename TI>
 static bool aligned4(const TI* p, int64_t ld, int64_t stride) {
@@ -561,7 +558,7 @@ static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double al
   if constexpr (TAG == 2 && !TB) {
     // the (T) W build: K = 2 (v + o) in two segments, output maps
     FH_CHECK(sg.A != nullptr, -3, "gemm_bf3: the (T) W build takes two K segments");
-    constexpr int TBK = FH_BF3_TBK;
+    constexpr int TBK = kTBK;
     if (bn == 64) {
       if (vec) {
         if (lm) FH_BF3_GO_N(TBK, 64, true, true, true); else FH_BF3_GO_N(TBK, 64, true, false, true);
