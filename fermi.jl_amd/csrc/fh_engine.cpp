@@ -860,7 +860,8 @@ __global__ void k_build_av(int o, int v, int ndp, bool swap_xy, const double* __
                            const double* __restrict__ t2, TO* __restrict__ av) {
   // AV[(x,y), n, d'] : d' < v  -> (n y | x d') = OVVV[n, y, x, d']   (ijk2.jl:33 vvvo layout)
   //                    v <= d' < v+o -> -T2[n, l, y, x], l = d' - v  (T2_n with the GEMM's minus sign)
-  //                    d' >= v+o -> 0 (K padded to ndp, a multiple of 4, for aligned vector loads)
+  //                    d' >= v+o -> 0 (K padded to ndp, a multiple of 8, for aligned vector loads and
+  //                    8-wide k chunks)
   // swap_xy stores the rows as (y,x) instead
   const int64_t vv = (int64_t)v * v, nd = (int64_t)v + o;
   const int64_t total = vv * o * ndp;
@@ -937,7 +938,7 @@ static double triples_t(Ctx& c, bool bf3) {
   FH_CHECK(c.have_amp, -6, "no amplitudes for (T)");
   const bool lowp = std::is_same<T, float>::value;
   const int64_t o = c.o, v = c.v, vv = v * v;
-  const int64_t nd = ((v + o + 3) / 4) * 4;  // K = v + o padded with zeros to a multiple of 4
+  const int64_t nd = ((v + o + 7) / 8) * 8;  // K = v + o padded with zeros to a multiple of 8
   const double o_ = (double)o, v_ = (double)v;
   const double ntrip = o_ * (o_ + 1) * (o_ + 2) / 6.0;
   const std::string sfx = bf3 ? "_bf16x3" : (lowp ? "_fp32" : "");

@@ -75,6 +75,17 @@ __device__ inline void put4(uint16_t* hi_plane, uint16_t* lo_plane, int off, con
   *reinterpret_cast<uint2*>(lo_plane + off) = make_uint2(l0, l1);
 }
 
+// split 8 values (k-consecutive) and write them to LDS rows (16 bytes per plane)
+__device__ inline void put8(uint16_t* hi_plane, uint16_t* lo_plane, int off, const V4& a, const V4& b) {
+  uint32_t h0, l0, h1, l1, h2, l2, h3, l3;
+  split2(a.x[0], a.x[1], h0, l0);
+  split2(a.x[2], a.x[3], h1, l1);
+  split2(b.x[0], b.x[1], h2, l2);
+  split2(b.x[2], b.x[3], h3, l3);
+  *reinterpret_cast<uint4*>(hi_plane + off) = make_uint4(h0, h1, h2, h3);
+  *reinterpret_cast<uint4*>(lo_plane + off) = make_uint4(l0, l1, l2, l3);
+}
+
 template <typename TI, bool VEC>
 __device__ inline V4 load4(const TI* p, bool full, int n_ok) {
   V4 v;
@@ -143,8 +154,8 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
   B += bz * sB;
   if (C) C += bz * sC;
   D += bz * sD;
-  // second K segment: for k >= K1 the operands are A2 / B2 at k - K1 (K1 % 4 == 0, so no 4-wide
-  // k group straddles the boundary)
+  // second K segment: for k >= K1 the operands are A2 / B2 at k - K1 (K1 % 8 == 0, checked at
+  // launch, so no 4-wide k group or 8-wide k chunk straddles the boundary)
   const TI* A2 = SEG ? static_cast<const TI*>(sg.A) + bz * sg.sA : A;
   const TI* B2 = SEG ? static_cast<const TI*>(sg.B) + bz * sg.sB : B;
   const int64_t K1 = SEG ? sg.K1 : K, lda2 = SEG ? sg.lda : lda, ldb2 = SEG ? sg.ldb : ldb;
@@ -189,18 +200,16 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
   // outside row reads a valid element and is zeroed), as offsets from the tile's first row
   // (row-contiguous) or as column offsets (k-contiguous).  A full k tile then costs one pointer
   // per k column (row-contiguous) or per row (k-contiguous) and an add per element.
-  constexpr int NA = TA ? KG : 4, NB = TB ? 4 : LB;
-  // row-contiguous: row index (M, N < 2^31, checked at launch); k-contiguous: ld * row, per segment
-  int arow[TA ? 1 : 4], brow[TB ? 4 : 1];
+  constexpr int NA = TA ? KG : 1, NB = TB ? 1 : LB;
+  // row-contiguous: the thread's row (M, N < 2^31, checked at launch); k-contiguous: ld * row,
+  // per segment
+  int arow = 0, brow = 0;
   int64_t aoff[TA ? KG : 1], aoff2[TA && SEG ? KG : 1], boff[TB ? 1 : LB], boff2[!TB && SEG ? LB : 1];
   bool ain[NA], bin[NB];
   if (!TA) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int64_t r = m0 + (tid & 31) + 32 * i;
-      ain[i] = r < M;
-      arow[i] = (int)(ain[i] ? r : M - 1);
-    }
+    const int64_t r = m0 + (tid & 127);
+    ain[0] = r < M;
+    arow = (int)(ain[0] ? r : M - 1);
   } else {
 #pragma unroll
     for (int j = 0; j < KG; ++j) {
@@ -211,12 +220,9 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
     }
   }
   if (TB) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int64_t r = n0 + (tid & 31) + 32 * i;
-      bin[i] = r < N;
-      brow[i] = (int)(bin[i] ? r : N - 1);
-    }
+    const int64_t r = n0 + (tid & 127);
+    bin[0] = r < N;
+    brow = (int)(bin[0] ? r : N - 1);
   } else {
 #pragma unroll
     for (int j = 0; j < LB; ++j) {
@@ -226,17 +232,24 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
       if constexpr (SEG) boff2[j] = ldb2 * (bin[j] ? r : N - 1);
     }
   }
-  // a k tile inside K (K1 % 4 == 0: no 4-wide k group straddles the segments); the values of
+  // row-contiguous pattern: row tid & 127 of the tile, 8-wide k chunks kc0 + 2 q (q < BK / 16):
+  // kc0 = tid >> 7 is the same for a whole wavefront, so a chunk's column pointer, segment and
+  // stride are uniform, the loads of one k are coalesced across the lanes' consecutive rows, and
+  // each lane stores 16 contiguous bytes per plane to LDS
+  const int kc0 = tid >> 7;
+  // a k tile inside K (K1 % 8 == 0: no 8-wide k chunk straddles the segments); the values of
   // outside rows are zeroed when the tile is stored to LDS (sstore), after the MFMAs of the
   // current tile, so that the loads stay in flight meanwhile
   auto gload_full = [&](int64_t k0) {
     if (!TA) {
 #pragma unroll
-      for (int j = 0; j < KG; ++j) {
-        const int64_t gk = k0 + KG * (tid >> 5) + j;
-        const TI* cp = (!SEG || gk < K1) ? A + lda * gk : A2 + lda2 * (gk - K1);
+      for (int q = 0; q < BK / 16; ++q) {
+        const int64_t gk = k0 + 8 * (kc0 + 2 * q);
+        const bool s1 = !SEG || gk < K1;
+        const TI* cp = (s1 ? A + lda * gk : A2 + lda2 * (gk - K1)) + arow;
+        const int64_t ld = s1 ? lda : lda2;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) ra[j].x[i] = (float)cp[arow[i]];
+        for (int e = 0; e < 8; ++e) ra[2 * q + (e >> 2)].x[e & 3] = (float)cp[ld * e];
       }
     } else {
       const int64_t gk = k0 + 4 * (tid % KQ);
@@ -248,11 +261,13 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
     }
     if (TB) {
 #pragma unroll
-      for (int j = 0; j < KG; ++j) {
-        const int64_t gk = k0 + KG * (tid >> 5) + j;
-        const TI* rp = (!SEG || gk < K1) ? B + ldb * gk : B2 + ldb2 * (gk - K1);
+      for (int q = 0; q < BK / 16; ++q) {
+        const int64_t gk = k0 + 8 * (kc0 + 2 * q);
+        const bool s1 = !SEG || gk < K1;
+        const TI* rp = (s1 ? B + ldb * gk : B2 + ldb2 * (gk - K1)) + brow;
+        const int64_t ld = s1 ? ldb : ldb2;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) rb[j].x[i] = (float)rp[brow[i]];
+        for (int e = 0; e < 8; ++e) rb[2 * q + (e >> 2)].x[e & 3] = (float)rp[ld * e];
       }
     } else {
       const int64_t gk = k0 + 4 * (tid % KQ);
@@ -267,13 +282,13 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
   auto gload_part = [&](int64_t k0) {
     if (!TA) {
 #pragma unroll
-      for (int j = 0; j < KG; ++j) {
-        const int64_t gk = k0 + KG * (tid >> 5) + j;
-        const bool kin = gk < K;
-        const TI* colp = aptr(0, kin ? gk : 0);
+      for (int q = 0; q < BK / 16; ++q)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) ra[j].x[i] = ld1(colp + arow[i], kin && ain[i]);
-      }
+        for (int e = 0; e < 8; ++e) {
+          const int64_t gk = k0 + 8 * (kc0 + 2 * q) + e;
+          const bool kin = gk < K;
+          ra[2 * q + (e >> 2)].x[e & 3] = ld1(aptr(0, kin ? gk : 0) + arow, kin);
+        }
     } else {
       const int64_t gk = k0 + 4 * (tid % KQ);
 #pragma unroll
@@ -284,13 +299,13 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
     }
     if (TB) {
 #pragma unroll
-      for (int j = 0; j < KG; ++j) {
-        const int64_t gk = k0 + KG * (tid >> 5) + j;
-        const bool kin = gk < K;
-        const TI* rowp = bptr(0, kin ? gk : 0);
+      for (int q = 0; q < BK / 16; ++q)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) rb[j].x[i] = ld1(rowp + brow[i], kin && bin[i]);
-      }
+        for (int e = 0; e < 8; ++e) {
+          const int64_t gk = k0 + 8 * (kc0 + 2 * q) + e;
+          const bool kin = gk < K;
+          rb[2 * q + (e >> 2)].x[e & 3] = ld1(bptr(0, kin ? gk : 0) + brow, kin);
+        }
     } else {
       const int64_t gk = k0 + 4 * (tid % KQ);
 #pragma unroll
@@ -304,18 +319,10 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
     if (k0 + BK <= K) gload_full(k0);
     else gload_part(k0);
   };
-  // r[j].x[i] = X[row (tid&31)+32i][k KG(tid>>5)+j]: gather the KG k of each row
+  // r[2q], r[2q+1] = X[row tid & 127][k 8 (kc0 + 2q) .. +7]
   auto put_rows = [&](uint16_t* hi, uint16_t* lo, const V4* r) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int off = ((tid & 31) + 32 * i) * kRow + KG * (tid >> 5);
-      if constexpr (KG == 4) {
-        const V4 t = {{r[0].x[i], r[1].x[i], r[2].x[i], r[3].x[i]}};
-        put4(hi, lo, off, t);
-      } else {
-        put2(hi, lo, off, r[0].x[i], r[1].x[i]);
-      }
-    }
+    for (int q = 0; q < BK / 16; ++q) put8(hi, lo, (tid & 127) * kRow + 8 * (kc0 + 2 * q), r[2 * q], r[2 * q + 1]);
   };
   auto put_k = [&](uint16_t* hi, uint16_t* lo, const V4* r, int nl) {
 #pragma unroll
@@ -328,7 +335,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
 #pragma unroll
       for (int j = 0; j < KG; ++j)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) ra[j].x[i] = keep(ra[j].x[i], ain[i]);
+        for (int i = 0; i < 4; ++i) ra[j].x[i] = keep(ra[j].x[i], ain[0]);
     } else {
 #pragma unroll
       for (int j = 0; j < KG; ++j)
@@ -339,7 +346,7 @@ __global__ __launch_bounds__(256, BK == 16 ? 3 : 2) void k_gemm_bf3(int64_t M, i
 #pragma unroll
       for (int j = 0; j < KG; ++j)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) rb[j].x[i] = keep(rb[j].x[i], bin[i]);
+        for (int i = 0; i < 4; ++i) rb[j].x[i] = keep(rb[j].x[i], bin[0]);
     } else {
 #pragma unroll
       for (int j = 0; j < LB; ++j)
@@ -558,6 +565,7 @@ static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double al
   if constexpr (TAG == 2 && !TB) {
     // the (T) W build: K = 2 (v + o) in two segments, output maps
     FH_CHECK(sg.A != nullptr, -3, "gemm_bf3: the (T) W build takes two K segments");
+    FH_CHECK(sg.K1 % 8 == 0, -3, "gemm_bf3: the first K segment must be a multiple of 8");
     constexpr int TBK = kTBK;
     if (bn == 64) {
       if (vec) {

@@ -89,7 +89,8 @@ struct OutMap {
 
 // Second K segment of a two-segment GEMM, D = alpha (A B + A2 B2) + beta C: for k >= K1 the
 // operands are A2 / B2 at k - K1 (same transposition, their own leading dimensions and batch
-// strides).  K1 must be a multiple of 4.  A == nullptr: one segment.
+// strides).  K1 must be a multiple of 4 (of 8 for the split-BF16x3 kernel).  A == nullptr: one
+// segment.
 struct KSeg {
   const void* A = nullptr;
   int64_t lda = 0, sA = 0;
