@@ -25,9 +25,10 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 PREC_NAMES = {"double": "FP64", "mixed": "FP32 MFMA + FP64 polish",
               "bf16x3": "split-BF16x3 MFMA + FP64 polish"}
 
-# HBM bytes per launch of the roofline kernels, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
-# scaled by a plain 1 GiB copy (FETCH x 2.0, WRITE x 1.0): profiles/triples_bf16x3_water10_hbm_counters.md
-MEASURED_TRAFFIC = {("water10", "triples_gemm_bf16x3"): 1.535e9}
+# HBM bytes per launch of the roofline kernels ((T) W build, PP ladder), from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE runs scaled by a plain 1 GiB copy (FETCH x 2.0, WRITE x 1.0):
+# profiles/triples_bf16x3_water10_hbm_counters.md
+MEASURED_TRAFFIC = {("water10", "triples_gemm_bf16x3"): 1.535e9, ("water10", "ladder_bf16x3"): 23.11e9}
 
 
 def log(*a):
