@@ -220,6 +220,8 @@ class Context:
         self._check(self.L.fh_prepare(self.h))
 
     def guess(self, kind=0, copy_out=True):
+        """Initial amplitudes (fh_t2_guess): kind 0 MP2, 1 learned (after set_t2_model); add 2 for
+        the first-order T1 instead of T1 = 0.  Kept on the device for solve(); returned when copy_out."""
         if not copy_out:
             self._check(self.L.fh_t2_guess(self.h, kind, None, None))
             return None, None
