@@ -114,7 +114,8 @@ int fh_ccsd_solve(fh_ctx* ctx, const fh_cc_opts* opts, double* T1, double* T2, f
 
 /* (T) correction RCCSDpT(ccsd, moints, ::ijk), ijk.jl:20-150.  prec_mode 0 = FP64; 1 = the W
  * build runs as FP32 MFMA with FP32 accumulation; 2 = the W build runs as split-BF16x3 MFMA
- * with FP32 accumulation.  V, denominators and the energy sum stay FP64 in every mode.
+ * with FP32 accumulation.  V is formed in FP64 (and kept in FP32 next to an FP32 W); the
+ * denominators and the energy sum stay FP64 in every mode.
  * T1/T2 NULL: use the device-resident amplitudes of the last solve. */
 int fh_triples(fh_ctx* ctx, int prec_mode, const double* T1, const double* T2, double* Et);
 
