@@ -584,9 +584,11 @@ static void launch_bf3(hipStream_t s, int64_t M, int64_t N, int64_t K, double al
     FH_CHECK(false, -3, "gemm_bf3: the (T) W build takes an untransposed B");
   } else {
     FH_CHECK(!sg.A, -3, "gemm_bf3: two-segment K is only instantiated for the (T) W build");
-    if (K <= 512) {
+    // BK = 16 (three workgroups per CU) for the contractions (tag 0; the o^3 v^3 ring GEMMs, K = o v,
+    // measured 3% faster than BK = 32) and short K; BK = 32 for the long-K PP ladder (tag 1, 5% faster)
+    if (TAG == 0 || K <= 512) {
       if (vec) FH_BF3_GO(16, true, true, false); else FH_BF3_GO(16, false, true, false);
-    } else {
+    } else if constexpr (TAG != 0) {
       if (vec) FH_BF3_GO(32, true, true, false); else FH_BF3_GO(32, false, true, false);
     }
   }
