@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -102,6 +103,7 @@ struct Ctx {
   std::unique_ptr<Pool> pool;
   int o = 0, v = 0, naux = 0;
   bool have_df = false, prepared = false;
+  bool ov_only = false;  // fh_set_df was given Bov alone (what Fermi's DF-MP2 computes, RMP2a.jl:49-54)
   int lowp = 0;  // residual contractions: 0 FP64 MFMA, 1 FP32 MFMA, 2 split-BF16x3 MFMA (large GEMMs)
   Buf eo, ev, Boo, Bov, Bvv;
   // 4-index blocks (Julia index order, column-major)
@@ -381,7 +383,10 @@ static void perm(Ctx& c, const TView& in, const std::string& inL, const TView& o
 }
 
 // ------------------------------------------------------------------ integrals
-static void require_df(Ctx& c) { FH_CHECK(c.have_df, -6, "no DF tensors: call fh_set_df first"); }
+static void require_df(Ctx& c, bool full = true) {
+  FH_CHECK(c.have_df, -6, "no DF tensors: call fh_set_df first");
+  FH_CHECK(!full || !c.ov_only, -6, "only Bov was given to fh_set_df: this entry point needs Boo and Bvv too");
+}
 
 static void prepare(Ctx& c) {
   require_df(c);
@@ -604,6 +609,40 @@ static void update_amp(Ctx& c, const double* t1, const double* t2, double* n1, d
   FH_HIP(hipStreamSynchronize(c.s));
   *r1 = std::sqrt(h[0] / (double)c.nov());
   *r2 = std::sqrt(h[1] / (double)c.n2());
+}
+
+// DF-MP2 correlation energy RMP2_energy (RMP2a.jl:91-144) from Bov and the orbital energies
+// alone.  Instead of the reference's loop over pairs i <= j with one v x v GEMM each, the
+// integrals X = (ia|jb) of a block of j come from one FP64 MFMA GEMM (K = naux) against a
+// [Q, b, j] copy of Bov, then k_mp2_pairs reduces them.  The blocks of j keep X under
+// FERMIHIP_MP2_BLOCK_MB MiB (default 4096; one block up to o v = 23 000).
+static double mp2_energy(Ctx& c) {
+  require_df(c, false);
+  const int keep = c.lowp;
+  c.lowp = 0;
+  const int64_t o = c.o, v = c.v, Q = c.naux, ov = c.nov();
+  Timer t(c, "df_mp2", 2.0 * (double)Q * (double)ov * (double)ov);
+  Buf Bvo = dalloc(c, Q * ov);
+  perm(c, tv(c.Bov.d(), {Q, o, v}), "Qjb", tv(Bvo.d(), {Q, v, o}), "Qbj", 1.0, 0.0);
+  int64_t cap_mb = 4096;
+  if (const char* env = std::getenv("FERMIHIP_MP2_BLOCK_MB")) cap_mb = std::max<int64_t>(1, std::atoll(env));
+  const int64_t per_j = ov * v * (int64_t)sizeof(double);
+  const int nj = (int)std::max<int64_t>(1, std::min<int64_t>(o, (cap_mb << 20) / per_j));
+  const int nblk = (int)((o + nj - 1) / nj);
+  Buf X = dalloc(c, ov * v * nj);
+  Buf part = dalloc(c, mp2_pairs_blocks((int)v, nj) * nblk);
+  int64_t np = 0;
+  for (int j0 = 0; j0 < o; j0 += nj) {
+    const int n = (int)std::min<int64_t>(nj, o - j0);
+    contract(c, 1.0, tv(c.Bov.d(), {Q, o, v}), "Qia", tv(Bvo.d() + Q * v * j0, {Q, v, (int64_t)n}), "Qbj", 0.0,
+             tv(X.d(), {o, v, v, (int64_t)n}), "iabj");
+    mp2_pairs(c.s, (int)o, (int)v, j0, n, X.d(), c.eo.d(), c.ev.d(), part.d() + np);
+    np += mp2_pairs_blocks((int)v, n);
+  }
+  if (!c.scal.p) c.scal = dalloc(c, 64);
+  sum_partials(c.s, np, part.d(), c.scal.d() + 2);
+  c.lowp = keep;
+  return read_scalar(c, c.scal.d() + 2);
 }
 
 static void mp2_guess(Ctx& c) {
@@ -1113,34 +1152,134 @@ void fh_destroy(fh_ctx* x) {
   (void)hipStreamDestroy(s);
 }
 
+// Forget the previous molecule (integral blocks, operand copies, amplitudes, guess network) and
+// take the orbital energies of the next one.
+static void new_molecule(Ctx& c, int o, int v, int naux, const double* eps_o, const double* eps_v, const char* who) {
+  for (int i = 0; i < o; ++i)
+    for (int a = 0; a < v; ++a)
+      FH_CHECK(eps_o[i] < eps_v[a], -1, std::string(who) + ": occupied orbital energy above a virtual one");
+  c.have_df = false;
+  c.prepared = false;
+  c.have_amp = false;
+  c.OOOO.reset(); c.OOOV.reset(); c.OOVV.reset(); c.OVOV.reset(); c.OVVV.reset(); c.Lovov.reset();
+  c.Vperm.reset(); c.Vperm32.reset(); c.Pt.reset(); c.T1.reset(); c.T2.reset();
+  c.Boo.reset(); c.Bov.reset(); c.Bvv.reset();
+  c.opcache.clear(); c.opcache_bytes = 0;
+  c.have_model = false; c.pair_d.reset();
+  c.o = o; c.v = v; c.naux = naux;
+  c.eo = dalloc(c, o); c.ev = dalloc(c, v);
+  h2d(c, c.eo.d(), eps_o, o);
+  h2d(c, c.ev.d(), eps_v, v);
+}
+
 int fh_set_df(fh_ctx* x, int o, int v, int naux, const double* eps_o, const double* eps_v, const double* Boo,
               const double* Bov, const double* Bvv) {
   return guarded([&] {
     FH_CHECK(x != nullptr, -1, "fh_set_df: null context");
     FH_CHECK(o >= 1 && v >= 1 && naux >= 1, -1, "fh_set_df: invalid dimensions");
-    FH_CHECK(eps_o && eps_v && Boo && Bov && Bvv, -1, "fh_set_df: null array");
+    FH_CHECK(eps_o && eps_v && Bov, -1, "fh_set_df: null array");
+    FH_CHECK((Boo == nullptr) == (Bvv == nullptr), -1, "fh_set_df: give both Boo and Bvv, or neither (DF-MP2 only)");
     DeviceGuard g(x->c.device);
     Ctx& c = x->c;
-    for (int i = 0; i < o; ++i)
-      for (int a = 0; a < v; ++a)
-        FH_CHECK(eps_o[i] < eps_v[a], -1, "fh_set_df: occupied orbital energy above a virtual one");
-    c.prepared = false;
-    c.have_amp = false;
-    c.OOOO.reset(); c.OOOV.reset(); c.OOVV.reset(); c.OVOV.reset(); c.OVVV.reset(); c.Lovov.reset();
-    c.Vperm.reset(); c.Vperm32.reset(); c.Pt.reset(); c.T1.reset(); c.T2.reset();
-    c.opcache.clear(); c.opcache_bytes = 0;
-    c.have_model = false; c.pair_d.reset();
-    c.o = o; c.v = v; c.naux = naux;
+    new_molecule(c, o, v, naux, eps_o, eps_v, "fh_set_df");
     const int64_t Q = naux;
-    c.eo = dalloc(c, o); c.ev = dalloc(c, v);
-    c.Boo = dalloc(c, Q * o * o); c.Bov = dalloc(c, Q * o * v); c.Bvv = dalloc(c, Q * v * v);
-    h2d(c, c.eo.d(), eps_o, o);
-    h2d(c, c.ev.d(), eps_v, v);
-    h2d(c, c.Boo.d(), Boo, Q * o * o);
+    c.ov_only = Boo == nullptr;
+    c.Bov = dalloc(c, Q * o * v);
     h2d(c, c.Bov.d(), Bov, Q * o * v);
-    h2d(c, c.Bvv.d(), Bvv, Q * v * v);
+    if (!c.ov_only) {
+      c.Boo = dalloc(c, Q * o * o); c.Bvv = dalloc(c, Q * v * v);
+      h2d(c, c.Boo.d(), Boo, Q * o * o);
+      h2d(c, c.Bvv.d(), Bvv, Q * v * v);
+    }
     FH_HIP(hipStreamSynchronize(c.s));
     c.have_df = true;
+  });
+}
+
+// AO -> MO DF transforms compute_BOO! / compute_BOV! / compute_BVV! (DFERI.jl:15-69) on the
+// device.  One FP64 MFMA GEMM makes the half-transformed H[Q,mu,q] = sum_nu b[Q,mu,nu] C[nu,q] for
+// the active MOs q (K = nbf), then three strided-batched GEMMs (one batch entry per second MO
+// index, K = nbf) give B[Q,p,q] = sum_mu H[Q,mu,q] C[mu,p] for the oo, ov and vv blocks.
+int fh_set_df_ao(fh_ctx* x, int nbf, int naux, int ndocc, int drop_occ, int drop_vir, const double* eps,
+                 const double* C, const double* bAO) {
+  return guarded([&] {
+    FH_CHECK(x != nullptr, -1, "fh_set_df_ao: null context");
+    FH_CHECK(eps && C && bAO, -1, "fh_set_df_ao: null array");
+    FH_CHECK(nbf >= 2 && naux >= 1 && ndocc >= 1 && ndocc < nbf, -1, "fh_set_df_ao: invalid dimensions");
+    // RMP2a.jl:62-68: frozen core below ndocc, inactive virtuals below nvir
+    FH_CHECK(drop_occ >= 0 && drop_occ < ndocc, -1,
+             "fh_set_df_ao: invalid number of frozen orbitals " + std::to_string(drop_occ) + " for " +
+                 std::to_string(ndocc) + " doubly occupied orbitals");
+    FH_CHECK(drop_vir >= 0 && drop_vir < nbf - ndocc, -1,
+             "fh_set_df_ao: invalid number of inactive virtual orbitals " + std::to_string(drop_vir) + " for " +
+                 std::to_string(nbf - ndocc) + " total virtual orbitals");
+    FH_CHECK((double)naux * nbf * nbf < 9.0e18 / 8, -1, "fh_set_df_ao: AO tensor too large");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    const int o = ndocc - drop_occ, v = nbf - drop_vir - ndocc, nact = o + v;
+    new_molecule(c, o, v, naux, eps + drop_occ, eps + ndocc, "fh_set_df_ao");  // Fock.jl:81-94
+    c.ov_only = false;
+    const int64_t Q = naux, n = nbf;
+    Buf dC = dalloc(c, n * n);
+    h2d(c, dC.d(), C, n * n);
+    Buf b = dalloc(c, Q * n * n);
+    h2d(c, b.d(), bAO, Q * n * n);
+    c.Boo = dalloc(c, Q * o * o);
+    c.Bov = dalloc(c, Q * o * v);
+    c.Bvv = dalloc(c, Q * v * v);
+    {
+      Timer t(c, "ao_to_mo", 2.0 * (double)Q * n * n * nact + 2.0 * (double)Q * n * ((double)o * o + (double)o * v +
+                                                                                        (double)v * v));
+      Buf H = dalloc(c, Q * n * nact);
+      {
+        const int64_t wsz = gemm_splitk_elems(Q * n, nact, n, 1);
+        Buf work;
+        if (wsz) work = dalloc(c, wsz);
+        gemm_f64(c.s, false, false, Q * n, nact, n, 1.0, b.d(), Q * n, 0, dC.d() + n * drop_occ, n, 0, 0.0, nullptr, 0,
+                 0, H.d(), Q * n, 0, 1, 0, work.p ? work.d() : nullptr, wsz);
+      }
+      b.reset();
+      const double* Co = dC.d() + n * drop_occ;
+      const double* Cv = dC.d() + n * ndocc;
+      auto half = [&](const double* Hq, int nq, const double* Cp, int np, double* B) {
+        const int64_t wsz = gemm_splitk_elems(Q, np, n, nq);
+        Buf work;
+        if (wsz) work = dalloc(c, wsz);
+        gemm_f64(c.s, false, false, Q, np, n, 1.0, Hq, Q, Q * n, Cp, n, 0, 0.0, nullptr, 0, 0, B, Q, Q * np, nq, 0,
+                 work.p ? work.d() : nullptr, wsz);
+      };
+      half(H.d(), o, Co, o, c.Boo.d());                   // Bij[Q,i,j]  (DFERI.jl:25-27)
+      half(H.d() + Q * n * o, v, Co, o, c.Bov.d());       // Bia[Q,i,a]  (DFERI.jl:46-48)
+      half(H.d() + Q * n * o, v, Cv, v, c.Bvv.d());       // Bab[Q,a,b]  (DFERI.jl:64-66)
+    }
+    FH_HIP(hipStreamSynchronize(c.s));
+    flush_timers(c);
+    c.have_df = true;
+  });
+}
+
+int fh_get_df(fh_ctx* x, double* Boo, double* Bov, double* Bvv) {
+  return guarded([&] {
+    FH_CHECK(x != nullptr, -1, "null context");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    require_df(c, false);
+    const int64_t Q = c.naux, o = c.o, v = c.v;
+    FH_CHECK(!(c.ov_only && (Boo || Bvv)), -6, "only Bov was given to fh_set_df");
+    if (Boo) d2h(c, Boo, c.Boo.d(), Q * o * o);
+    if (Bov) d2h(c, Bov, c.Bov.d(), Q * o * v);
+    if (Bvv) d2h(c, Bvv, c.Bvv.d(), Q * v * v);
+  });
+}
+
+int fh_mp2_energy(fh_ctx* x, double* E) {
+  return guarded([&] {
+    FH_CHECK(x != nullptr && E != nullptr, -1, "null argument");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    *E = mp2_energy(c);
+    flush_timers(c);
+    FH_CHECK(std::isfinite(*E), -5, "non-finite energy");
   });
 }
 

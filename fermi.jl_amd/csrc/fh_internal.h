@@ -185,6 +185,12 @@ void cast_f64_to_f32(hipStream_t s, int64_t n, const double* x, float* y);
 void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, const double* ovov,
                double* partials, double* result);
 
+// DF-MP2 energy of one block of j (RMP2a.jl:112-138): X[(i + o a) + o v (b + v (j - j0))] = (ia|jb)
+// for j in [j0, j0 + nj) -> mp2_pairs_blocks(v, nj) FP64 partial sums
+int64_t mp2_pairs_blocks(int v, int nj);
+void mp2_pairs(hipStream_t s, int o, int v, int j0, int nj, const double* X, const double* eo, const double* ev,
+               double* partials);
+
 // (T) energy for one (i,j) pair and k in [k0, k0+nk): W[a + v*b + v^2*c + v^3*(k-k0)] holds the
 // W intermediate of ijk.jl:108-115; one FP64 partial per workgroup.
 // orbits: device list of cube-block triples (A>=B>=C), 3 ints each (triples_orbits)

@@ -321,6 +321,41 @@ void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, 
   sum_partials(s, g, partials, result);
 }
 
+// ------------------------------------------------------------------ DF-MP2 pair energies (RMP2a.jl:112-138)
+// X[r + o v c] = (ia|jb) for rows r = i + o a and columns c = b + v (j - j0) of one block of j.
+// E += X[ia,jb] (2 X[ia,jb] - X[ib,ja]) / (e_i + e_j - e_a - e_b) summed over all ordered (i, j),
+// which equals the reference's i <= j sum with weight 2 for i != j (the (j, i) summand is the (i, j)
+// one with a and b relabelled).  The partner X[ib,ja] = X[(i + o b) + o v (a + v (j - j0))] lies in
+// the same block of j; along a column, consecutive threads read runs of o consecutive partners.
+// One workgroup per column (grid-strided); one FP64 partial per workgroup.
+__global__ __launch_bounds__(RB) void k_mp2_pairs(int o, int v, int j0, int ncol, const double* __restrict__ X,
+                                                  const double* __restrict__ eo, const double* __restrict__ ev,
+                                                  double* __restrict__ partials) {
+  const int ov = o * v;
+  double acc = 0.0;
+  for (int c = blockIdx.x; c < ncol; c += gridDim.x) {
+    const int b = c % v, jj = c / v;
+    const double ejb = eo[j0 + jj] - ev[b];
+    const double* __restrict__ xc = X + (int64_t)ov * c;
+    const double* __restrict__ xp = X + (int64_t)o * b + (int64_t)ov * v * jj;  // + i + ov * a
+    for (int r = threadIdx.x; r < ov; r += RB) {
+      const int i = r % o, a = r / o;
+      const double x = xc[r];
+      const double xt = xp[i + (int64_t)ov * a];
+      acc += x * (2.0 * x - xt) / (eo[i] - ev[a] + ejb);
+    }
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+int64_t mp2_pairs_blocks(int v, int nj) { return std::min<int64_t>((int64_t)v * nj, 2048); }
+void mp2_pairs(hipStream_t s, int o, int v, int j0, int nj, const double* X, const double* eo, const double* ev,
+               double* partials) {
+  const int64_t g = mp2_pairs_blocks(v, nj);
+  hipLaunchKernelGGL(k_mp2_pairs, dim3(g), dim3(RB), 0, s, o, v, j0, v * nj, X, eo, ev, partials);
+  FH_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ (T) energy epilogue (ijk.jl:116-136)
 // For one occupied triple the reference's restricted sum over a>=b>=c with weights
 // 1/(1+d_ab+d_bc) equals the full sum over ordered (a,b,c) of

@@ -7,6 +7,8 @@ C ABI in ``include/fermihip.h``:
   * ``RCCSD(moints, alg=RCCSDhip())``               -> MP2 guess + iterations  (RCCSDa.jl:46-90)
   * ``RCCSD(moints, T1, T2, alg=RCCSDhip())``       -> iterations from given amplitudes (RCCSDa.jl:92-245)
   * ``RCCSDpT(ccsd, moints, alg=ijkhip())``         -> (T) correction (ijk.jl:20-150)
+  * ``RMP2(moints, alg=RMP2hip())``                 -> DF-MP2 energy (RMP2a.jl:56-144); only
+    ``BOV`` and the orbital energies are needed (``BOO``/``BVV`` may be None)
 
 ``moints`` is a :class:`MOIntegrals`, the subset of Fermi's ``IntegralHelper`` cache the path
 reads: ``BOO[Q,o,o]``, ``BOV[Q,o,v]``, ``BVV[Q,v,v]``, ``Fii[o]``, ``Faa[v]`` and the
@@ -24,13 +26,13 @@ import numpy as np
 
 __all__ = [
     "FermiException", "MOIntegrals", "Options", "RCCSDhip", "ijkhip", "RCCSD", "RCCSDpT", "Context",
-    "lib_path", "load_library", "EXPORTED_SYMBOLS", "Mlp", "PRECISIONS",
+    "lib_path", "load_library", "EXPORTED_SYMBOLS", "Mlp", "PRECISIONS", "RMP2", "RMP2hip",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXPORTED_SYMBOLS = [
     "fh_default_opts", "fh_create", "fh_set_df", "fh_prepare", "fh_t2_guess", "fh_ccsd_energy",
-    "fh_ccsd_residual", "fh_ccsd_solve", "fh_triples", "fh_stats", "fh_last_error", "fh_destroy",
+    "fh_ccsd_residual", "fh_ccsd_solve", "fh_mp2_energy", "fh_set_df_ao", "fh_get_df", "fh_triples", "fh_stats", "fh_last_error", "fh_destroy",
     "fh_mlp_create", "fh_mlp_num_params", "fh_mlp_get_params", "fh_mlp_set_params", "fh_mlp_forward",
     "fh_mlp_loss_grad", "fh_mlp_train", "fh_mlp_destroy", "fh_set_t2_model",
 ]
@@ -86,6 +88,9 @@ def load_library():
     L.fh_ccsd_residual.argtypes = [ctypes.c_void_p, dp, dp, dp, dp]
     L.fh_ccsd_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Opts), dp, dp, ctypes.POINTER(_Result)]
     L.fh_triples.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp]
+    L.fh_mp2_energy.argtypes = [ctypes.c_void_p, dp]
+    L.fh_set_df_ao.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [dp, dp, dp]
+    L.fh_get_df.argtypes = [ctypes.c_void_p, dp, dp, dp]
     L.fh_stats.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     L.fh_last_error.argtypes = []
     L.fh_last_error.restype = ctypes.c_char_p
@@ -208,16 +213,45 @@ class Context:
             pass
 
     def set_df(self, eps_o, eps_v, Boo, Bov, Bvv):
+        """Hand over the DF tensors (Boo and Bvv may both be None: a context for mp2_energy only)."""
         Q, o, v = Bov.shape
-        if Boo.shape != (Q, o, o) or Bvv.shape != (Q, v, v) or len(eps_o) != o or len(eps_v) != v:
+        if (Boo is None) != (Bvv is None):
+            raise FermiException("give both BOO and BVV, or neither (DF-MP2 only)")
+        if (Boo is not None and (Boo.shape != (Q, o, o) or Bvv.shape != (Q, v, v))) or len(eps_o) != o \
+                or len(eps_v) != v:
             raise FermiException("inconsistent DF tensor shapes")
-        self._keep = [_colmajor(x) for x in (eps_o, eps_v, Boo, Bov, Bvv)]
+        self._keep = [None if x is None else _colmajor(x) for x in (eps_o, eps_v, Boo, Bov, Bvv)]
         self._check(self.L.fh_set_df(self.h, o, v, Q, *[_ptr(x) for x in self._keep]))
         self._keep = None
         self.o, self.v, self.naux = o, v, Q
 
+    def set_df_ao(self, eps, C, bAO, ndocc, drop_occ=0, drop_vir=0):
+        """AO -> MO DF transforms on the device (fh_set_df_ao, DFERI.jl:15-69): bAO[Q,mu,nu] the fitted
+        AO tensor, C[mu,p] and eps[p] the RHF orbitals, ndocc doubly occupied orbitals of which the
+        first drop_occ are frozen, the last drop_vir virtuals inactive."""
+        Q, n, n2 = bAO.shape
+        if n2 != n or C.shape != (n, n) or len(eps) != n:
+            raise FermiException("inconsistent AO tensor / orbital shapes")
+        keep = [_colmajor(x) for x in (eps, C, bAO)]
+        self._check(self.L.fh_set_df_ao(self.h, n, Q, int(ndocc), int(drop_occ), int(drop_vir),
+                                        *[_ptr(x) for x in keep]))
+        self.o, self.v, self.naux = ndocc - drop_occ, n - drop_vir - ndocc, Q
+
+    def get_df(self):
+        """The MO DF tensors held by the context: (BOO[Q,o,o], BOV[Q,o,v], BVV[Q,v,v])."""
+        Q, o, v = self.naux, self.o, self.v
+        out = [np.zeros(s, order="F") for s in ((Q, o, o), (Q, o, v), (Q, v, v))]
+        self._check(self.L.fh_get_df(self.h, *[_ptr(x) for x in out]))
+        return tuple(out)
+
     def prepare(self):
         self._check(self.L.fh_prepare(self.h))
+
+    def mp2_energy(self):
+        """DF-MP2 correlation energy (fh_mp2_energy, RMP2a.jl:91-144)."""
+        E = ctypes.c_double()
+        self._check(self.L.fh_mp2_energy(self.h, ctypes.byref(E)))
+        return E.value
 
     def guess(self, kind=0, copy_out=True):
         """Initial amplitudes (fh_t2_guess): kind 0 MP2, 1 learned (after set_t2_model); add 2 for
@@ -341,6 +375,28 @@ def _ctx_for(moints, device=0):
         moints.context = Context(device)
         moints.context.set_df(moints.Fii, moints.Faa, moints.BOO, moints.BOV, moints.BVV)
     return moints.context
+
+
+class RMP2hip:
+    """RMP2 algorithm singleton for the MI355X backend (the role of RMP2a, RMP2.jl:69-72)."""
+
+
+@dataclass
+class RMP2Result:
+    """Fields of Fermi's RMP2{T} (RMP2.jl:50-53)."""
+    correlation: float
+    energy: float
+
+
+def RMP2(moints, alg=None, device=0):
+    """RMP2(moints; alg) -> RMP2Result (RMP2a.jl:56-88 with the DF energy of RMP2a.jl:91-144).
+    The DF tensors carry the active spaces already (drop_occ / drop_vir applied when BOV was made,
+    DFERI.jl:31-51); energy = correlation + moints.sd_energy (RMP2a.jl:80-88)."""
+    if alg is not None and not isinstance(alg, RMP2hip):
+        raise FermiException(f"invalid algorithm {type(alg).__name__} for the MI355X RMP2 backend")
+    ctx = _ctx_for(moints, device)
+    Ec = ctx.mp2_energy()
+    return RMP2Result(Ec, Ec + moints.sd_energy)
 
 
 def RCCSD(moints, *args, alg=None, options=None, device=0):

@@ -80,9 +80,30 @@ int fh_create(fh_ctx** ctx, int device);
 /* Hand over the DF tensors and orbital energies (copied to HBM).
  * eps_o[o] = moints["Fii"], eps_v[v] = moints["Faa"] (Fock.jl:93-94);
  * Boo[Q,o,o], Bov[Q,o,v], Bvv[Q,v,v] = moints["BOO"/"BOV"/"BVV"] (DFERI.jl:15-69).
- * Replaces the compute! calls of RCCSD(moints, aoints, ::RCCSDa), RCCSDa.jl:37-44. */
+ * Replaces the compute! calls of RCCSD(moints, aoints, ::RCCSDa), RCCSDa.jl:37-44.
+ * Boo and Bvv may both be NULL: a context for fh_mp2_energy only, like the DF RMP2 driver, which
+ * computes BOV alone (RMP2a.jl:49-54); the CCSD and (T) entry points then fail with -6. */
 int fh_set_df(fh_ctx* ctx, int o, int v, int naux, const double* eps_o, const double* eps_v, const double* Boo,
               const double* Bov, const double* Bvv);
+
+/* AO -> MO DF transforms on the device, then as fh_set_df: compute_BOO! / compute_BOV! /
+ * compute_BVV! (DFERI.jl:15-69) and the RHF orbital energies of compute_F! (Fock.jl:81-98).
+ * bAO[Q,mu,nu] = aoints["ERI"], the fitted AO tensor b = (mu nu|P) J^-1/2 (AtomicIntegrals.jl:31-41);
+ * C[mu,p] = orbitals.C and eps[p] = orbitals.eps (nbf x nbf and nbf, column-major); ndocc = Nalpha;
+ * drop_occ / drop_vir = Options "drop_occ" / "drop_vir": occupied o = drop_occ+1 : ndocc and
+ * virtual ndocc+1 : nbf-drop_vir (1-based, DFERI.jl:20,39-40).  FP64 MFMA GEMMs. */
+int fh_set_df_ao(fh_ctx* ctx, int nbf, int naux, int ndocc, int drop_occ, int drop_vir, const double* eps,
+                 const double* C, const double* bAO);
+
+/* Copy the context's MO DF tensors back to the host (the moints cache entries "BOO", "BOV", "BVV"
+ * that DFERI.jl:29,50,68 store); any pointer may be NULL. */
+int fh_get_df(fh_ctx* ctx, double* Boo, double* Bov, double* Bvv);
+
+/* DF-MP2 correlation energy RMP2_energy(ints::IntegralHelper{T,<:AbstractDFERI,RHFOrbitals}, ...)
+ * (RMP2a.jl:91-144): sum over i, j, a, b of (ia|jb) (2 (ia|jb) - (ib|ja)) / (e_i + e_j - e_a - e_b)
+ * from the context's Bov and orbital energies (FP64 throughout).  Add the reference energy on the
+ * host (RMP2a.jl:80-88).  The MP2 amplitudes are fh_t2_guess kind 0. */
+int fh_mp2_energy(fh_ctx* ctx, double* Emp2);
 
 /* Assemble the 4-index integral blocks on the device (DFERI.jl:71-197).  Optional: the
  * other entry points call it on first use. */
