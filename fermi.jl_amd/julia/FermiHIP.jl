@@ -7,7 +7,9 @@ Julia side of the MI355X DF-RCCSD(T) backend: registers `RCCSDhip <: RCCSDAlgori
 
 Selection: `@set cc_alg 2` picks `RCCSDhip` through `get_rccsd_alg(Val(2))` (the mechanism of
 Fermi's RCCSD.jl:76-88); `@set pt_alg 4` picks `ijkhip` through the `get_rpt_alg` method
-below (PerturbativeTriples.jl:3-11 keeps a fixed vector, so this module redefines it).
+below (PerturbativeTriples.jl:3-11 keeps a fixed vector, so this module redefines it);
+`@set mp2_alg 3` picks `RMP2hip` through `get_rmp2_alg(Val(3))` (RMP2.jl:60-77), whose DF energy
+`RMP2_energy` (the method RMP2a.jl:80 calls) runs on the device.
 """
 module FermiHIP
 
@@ -16,13 +18,17 @@ using Fermi.Options
 using Fermi.Integrals: IntegralHelper, AbstractDFERI
 using Fermi.CoupledCluster: RCCSDAlgorithm, RpTAlgorithm, ijk, ijk2, abc
 import Fermi.CoupledCluster: RCCSD, RCCSDpT, get_rccsd_alg, get_rpt_alg
+using Fermi.MollerPlesset: RMP2Algorithm
+import Fermi.MollerPlesset: RMP2_energy, get_rmp2_alg
 
 const libfermihip = get(ENV, "FERMIHIP_LIB", joinpath(@__DIR__, "..", "fermihip", "libfermihip.so"))
 
 struct RCCSDhip <: RCCSDAlgorithm end
 struct ijkhip <: RpTAlgorithm end
+struct RMP2hip <: RMP2Algorithm end
 
 get_rccsd_alg(::Val{2}) = RCCSDhip()
+get_rmp2_alg(::Val{3}) = RMP2hip()
 function get_rpt_alg()
     N = Options.get("pt_alg")
     N == 4 && return ijkhip()
@@ -108,13 +114,55 @@ function cc_opts()
            0, -1, 1e-9, 1e-9, 1e-7, 1e-8)   # FP64 contractions; the mixed modes are selected by the Python/C callers
 end
 
-# RCCSD(moints, aoints, ::RCCSDa) for DF ERIs (RCCSDa.jl:37-44): only the three B blocks and F are needed
+# RCCSD(moints, aoints, ::RCCSDa) for DF ERIs (RCCSDa.jl:37-44): only the three B blocks and F are
+# needed.  The AO -> MO transforms (DFERI.jl:15-69) run on the device unless the blocks are cached
+# already; the MO blocks are copied back into the moints cache, where compute_BOO!/BOV!/BVV! put them.
 function RCCSD(moints::IntegralHelper{Float64,E1,O}, aoints::IntegralHelper{Float64,E2,Fermi.Orbitals.AtomicOrbitals},
                alg::RCCSDhip) where {E1<:AbstractDFERI,E2<:AbstractDFERI,O}
-    for key in ("F", "BOO", "BOV", "BVV")
-        Fermi.Integrals.compute!(moints, aoints, key)
+    Fermi.Integrals.compute!(moints, aoints, "F")
+    if all(k -> haskey(moints.cache, k), ("BOO", "BOV", "BVV"))
+        context_for(moints)
+    else
+        transform_df!(moints, aoints)
     end
     RCCSD(moints, alg)
+end
+
+function transform_df!(moints::IntegralHelper{Float64}, aoints::IntegralHelper{Float64})
+    b = Array{Float64}(aoints["ERI"])
+    C = Array{Float64}(moints.orbitals.C)
+    eps = Array{Float64}(moints.orbitals.eps)
+    Q, nbf, _ = size(b)
+    ndocc = moints.molecule.Nα
+    core, inac = Options.get("drop_occ"), Options.get("drop_vir")
+    ctx = Context(0)
+    GC.@preserve b C eps begin
+        check(ccall((:fh_set_df_ao, libfermihip), Cint,
+                    (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                    ctx.h, nbf, Q, ndocc, core, inac, eps, C, b))
+    end
+    o, v = ndocc - core, nbf - inac - ndocc
+    Boo, Bov, Bvv = zeros(Float64, Q, o, o), zeros(Float64, Q, o, v), zeros(Float64, Q, v, v)
+    check(ccall((:fh_get_df, libfermihip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                ctx.h, Boo, Bov, Bvv))
+    moints["BOO"], moints["BOV"], moints["BVV"] = Boo, Bov, Bvv
+    CONTEXTS[moints] = ctx
+    return ctx
+end
+
+# DF-MP2 energy (RMP2a.jl:91-144) on the device: an MP2-only context holds BOV and the orbital energies
+function RMP2_energy(ints::IntegralHelper{Float64,<:AbstractDFERI,Fermi.Orbitals.RHFOrbitals}, alg::RMP2hip)
+    Bov, fo, fv = ints["BOV"], ints["Fii"], ints["Faa"]
+    Q, o, v = size(Bov)
+    ctx = Context(0)
+    E = Ref{Float64}(0.0)
+    GC.@preserve Bov fo fv begin
+        check(ccall((:fh_set_df, libfermihip), Cint,
+                    (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                    ctx.h, o, v, Q, fo, fv, C_NULL, Bov, C_NULL))
+    end
+    check(ccall((:fh_mp2_energy, libfermihip), Cint, (Ptr{Cvoid}, Ref{Float64}), ctx.h, E))
+    return E[]
 end
 
 # MP2 guess on the device (RCCSDa.jl:46-90)
