@@ -338,6 +338,26 @@ def main():
                  "frac_fp64_mfma_peak": tf / FP64_MFMA_PEAK_TFLOPS}
         log(f"bench: AO->MO DF transform nbf={nbf} {df_ao['ms']:.2f} ms, {tf:.1f} TFLOP/s")
     del b_ao
+    # DF-RHF SCF (fh_rhf, FP64) at the same AO shape: synthetic S, H, b; one untimed call first
+    S_s, H_s, b_s = synth.scf_like(nbf, Q, args.seed)
+    ctx.rhf(S_s, H_s, b_s, o + ncore)
+    b_ph = ctx.stats()["phases"]
+    t0 = time.perf_counter()
+    r_scf, _, _ = ctx.rhf(S_s, H_s, b_s, o + ncore)
+    t_scf = time.perf_counter() - t0
+    ph_s = phase_diff(ctx.stats()["phases"], b_ph)
+    df_rhf = None
+    if ph_s.get("scf") and ph_s.get("scf_fock"):
+        fk, dg = ph_s["scf_fock"], ph_s["scf_diag"]
+        df_rhf = {"nbf": nbf, "naux": Q, "ndocc": o + ncore, "niter": r_scf["niter"], "converged": r_scf["converged"],
+                  "energy": r_scf["energy"], "device_ms": ph_s["scf"]["ms"], "pcie_inclusive_s": t_scf,
+                  "fock_ms_per_call": fk["ms"] / max(1, fk["calls"]),
+                  "fock_tflops": fk["flops"] / (fk["ms"] * 1e-3) / 1e12,
+                  "diag_ms_per_call": dg["ms"] / max(1, dg["calls"])}
+        log(f"bench: DF-RHF nbf={nbf} {r_scf['niter']} iterations, {ph_s['scf']['ms']:.1f} ms on the device "
+            f"(Fock {df_rhf['fock_ms_per_call']:.2f} ms/call at {df_rhf['fock_tflops']:.1f} TFLOP/s, diagonalisation "
+            f"{df_rhf['diag_ms_per_call']:.2f} ms/call)")
+    del b_s
     out = {
         "metric": "DF-RCCSD(T) time-to-solution (s) + MFMA TFLOP/s at <=1 uEh vs FP64 GPU path",
         "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -361,6 +381,7 @@ def main():
         "kernels": {k: vv for k, vv in kern.items() if vv},
         "df_mp2": df_mp2,
         "df_ao_to_mo": df_ao,
+        "df_rhf": df_rhf,
     }
     if args.cpu_baseline == "auto":
         log("bench: timing CPU oracle sample")

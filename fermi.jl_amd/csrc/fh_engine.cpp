@@ -9,6 +9,9 @@
 #include "fh_internal.h"
 #include "../../include/fermihip.h"
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -779,6 +782,166 @@ struct DIIS {
   }
 };
 
+// ------------------------------------------------------------------ DF-RHF SCF (RHFa.jl:67-223)
+// Symmetric eigensolver on the device (rocSOLVER syevd, the divide-and-conquer LAPACK method):
+// A (n x n, column-major) is overwritten by the eigenvectors, w gets the eigenvalues ascending, as
+// eigen(Symmetric(A), sortby = x -> x) orders them.
+struct Eigen {
+  Ctx& c;
+  rocblas_handle h = nullptr;
+  Buf work, info;
+  int n;
+  Eigen(Ctx& cc, int nn) : c(cc), n(nn) {
+    FH_CHECK(rocblas_create_handle(&h) == rocblas_status_success, -2, "rocblas_create_handle failed");
+    FH_CHECK(rocblas_set_stream(h, c.s) == rocblas_status_success, -2, "rocblas_set_stream failed");
+    work = dalloc(c, n);
+    info = dalloc(c, 1);
+  }
+  ~Eigen() { if (h) (void)rocblas_destroy_handle(h); }
+  void run(double* A, double* w) {
+    FH_CHECK(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, n, w, work.d(),
+                              reinterpret_cast<rocblas_int*>(info.p)) == rocblas_status_success, -2,
+             "rocsolver_dsyevd failed");
+    rocblas_int hi = 0;
+    FH_HIP(hipMemcpyAsync(&hi, info.p, sizeof(hi), hipMemcpyDeviceToHost, c.s));
+    FH_HIP(hipStreamSynchronize(c.s));
+    FH_CHECK(hi == 0, -5, "eigensolver did not converge (info " + std::to_string(hi) + ")");
+  }
+};
+
+// C = alpha op(A) op(B) + beta C for the SCF's small dense matrices (FP64 MFMA; split-K workspace
+// from the pool when the shape asks for it)
+static void mm(Ctx& c, bool ta, bool tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+               const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+  const int64_t wsz = gemm_splitk_elems(M, N, K, 1);
+  Buf work;
+  if (wsz) work = dalloc(c, wsz);
+  gemm_f64(c.s, ta, tb, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, beta != 0.0 ? C : nullptr, ldc, 0, C, ldc, 0, 1,
+           0, work.p ? work.d() : nullptr, wsz);
+}
+
+static double dotd(Ctx& c, int64_t n, const double* x, const double* y, int slot) {
+  dot_f64(c.s, n, x, y, c.partials.d(), c.scal.d() + slot);
+  return read_scalar(c, c.scal.d() + slot);
+}
+
+// out = in^T (n x n)
+static void transpose_sq(Ctx& c, int64_t n, const double* in, double alpha, double beta, double* out) {
+  const int64_t od[2] = {n, n}, is[2] = {n, 1};
+  permute_f64(c.s, 2, od, is, alpha, in, beta, out);
+}
+
+// RHF(ints, C, Lambda, ::RHFa) with DF integrals: S, H = T + V [n,n], b[Q,mu,nu] in HBM.
+static void rhf(Ctx& c, const fh_scf_opts& op, int n, int naux, int no, double Vnuc, const double* S, const double* H,
+                const double* b, double* eps_out, double* C_out, fh_scf_result& res) {
+  const int64_t nn = (int64_t)n * n, Q = naux;
+  if (!c.partials.p) c.partials = dalloc(c, 4096);
+  if (!c.scal.p) c.scal = dalloc(c, 64);
+  Eigen eig(c, n);
+  Buf L = dalloc(c, nn), F = dalloc(c, nn), Ft = dalloc(c, nn), Cm = dalloc(c, nn), D = dalloc(c, nn),
+      Dold = dalloc(c, nn), Ftil = dalloc(c, nn), Dtil = dalloc(c, nn), T1 = dalloc(c, nn), T2 = dalloc(c, nn),
+      J = dalloc(c, nn), K = dalloc(c, nn), w = dalloc(c, n), X = dalloc(c, Q), Z = dalloc(c, Q * no * n);
+  // Lambda = S^-1/2 = U diag(d)^-1/2 U^T (RHFHelper.jl:9,36)
+  {
+    Timer t(c, "scf_guess");
+    d2d(c, T1.d(), S, nn);
+    eig.run(T1.d(), w.d());
+    double dmin = 0.0;
+    FH_HIP(hipMemcpyAsync(&dmin, w.d(), sizeof(double), hipMemcpyDeviceToHost, c.s));
+    FH_HIP(hipStreamSynchronize(c.s));
+    FH_CHECK(dmin > 0.0, -1, "fh_rhf: overlap matrix is not positive definite");
+    scale_cols_rsqrt(c.s, n, T1.d(), w.d(), T2.d());
+    mm(c, false, true, n, n, n, 1.0, T2.d(), n, T1.d(), n, 0.0, L.d(), n);
+    // guess Fock matrix: GWH (RHFHelper.jl:44-51) or the core Hamiltonian (RHFHelper.jl:10)
+    if (op.guess == 0) gwh_fock(c.s, n, S, H, F.d());
+    else d2d(c, F.d(), H, nn);
+  }
+  // Ft = L^T F L; eps, Ct = eigen(Ft); C = L Ct; D = Co Co^T
+  auto orbitals = [&](const double* Fm) {
+    Timer t(c, "scf_diag", 4.0 * (double)nn * n + 2.0 * (double)nn * no);
+    mm(c, false, false, n, n, n, 1.0, Fm, n, L.d(), n, 0.0, T1.d(), n);
+    mm(c, true, false, n, n, n, 1.0, L.d(), n, T1.d(), n, 0.0, Ft.d(), n);
+    eig.run(Ft.d(), w.d());
+    mm(c, false, false, n, n, n, 1.0, L.d(), n, Ft.d(), n, 0.0, Cm.d(), n);
+    mm(c, false, true, n, n, no, 1.0, Cm.d(), n, Cm.d(), n, 0.0, D.d(), n);
+  };
+  // build_fock! with DF integrals (RHFHelper.jl:88-93): F = H + 2 J - K with
+  // J[m,n] = b[Q,m,n] (b[Q,r,s] D[r,s]) and, since D = Co Co^T, K[m,n] = Z[Q,i,m] Z[Q,i,n] with
+  // Z[Q,i,m] = b[Q,m,r] Co[r,i] (one strided-batched GEMM over m)
+  auto fock = [&]() {
+    Timer t(c, "scf_fock", 4.0 * (double)Q * nn + 2.0 * (double)Q * nn * no + 2.0 * (double)nn * Q * no);
+    mm(c, false, false, Q, 1, nn, 1.0, b, Q, D.d(), nn, 0.0, X.d(), Q);
+    mm(c, true, false, nn, 1, Q, 1.0, b, Q, X.d(), Q, 0.0, J.d(), nn);
+    const int64_t wsz = gemm_splitk_elems(Q, no, n, n);
+    Buf work;
+    if (wsz) work = dalloc(c, wsz);
+    gemm_f64(c.s, false, false, Q, no, n, 1.0, b, Q * (int64_t)n, Q, Cm.d(), n, 0, 0.0, nullptr, 0, 0, Z.d(), Q,
+             Q * no, n, 0, work.p ? work.d() : nullptr, wsz);
+    mm(c, true, false, n, n, Q * no, 1.0, Z.d(), Q * no, Z.d(), Q * no, 0.0, K.d(), n);
+    d2d(c, F.d(), H, nn);
+    axpby_f64(c.s, nn, 2.0, J.d(), 1.0, F.d());
+    axpby_f64(c.s, nn, -1.0, K.d(), 1.0, F.d());
+  };
+  orbitals(F.d());
+  // the loop starts from the guess orbitals' density (RHFa.jl:118-127)
+  fock();
+  d2d(c, Ftil.d(), F.d(), nn);
+  d2d(c, Dtil.d(), D.d(), nn);
+  d2d(c, Dold.d(), D.d(), nn);
+  std::unique_ptr<DIIS> dm;
+  if (op.diis) dm.reset(new DIIS(c, nn, op.ndiis, false));
+  int ite = 1;
+  double E = 0.0, dE = 1.0, Drms = 1.0;
+  bool converged = false;
+  while (ite <= op.scf_max_iter) {
+    orbitals(F.d());                                              // RHFa.jl:137-147
+    fock();                                                       // :150
+    const double Enew = dotd(c, nn, D.d(), H, 8) + dotd(c, nn, D.d(), F.d(), 9) + Vnuc;   // :151-154
+    if (dm) {                                                     // err = L^T (F D S - S D F) L  (:157-160)
+      mm(c, false, false, n, n, n, 1.0, F.d(), n, D.d(), n, 0.0, T1.d(), n);       // F D
+      mm(c, false, false, n, n, n, 1.0, T1.d(), n, S, n, 0.0, T2.d(), n);          // F D S
+      mm(c, false, false, n, n, n, 1.0, S, n, D.d(), n, 0.0, J.d(), n);            // S D (J is free here)
+      mm(c, false, false, n, n, n, -1.0, J.d(), n, F.d(), n, 1.0, T2.d(), n);      // - S D F
+      mm(c, false, false, n, n, n, 1.0, T2.d(), n, L.d(), n, 0.0, T1.d(), n);
+      Buf err = dalloc(c, nn);
+      mm(c, true, false, n, n, n, 1.0, L.d(), n, T1.d(), n, 0.0, err.d(), n);
+      dm->push(F.d(), std::move(err));
+    }
+    if (op.oda && Drms > op.oda_cutoff && ite < op.oda_shutoff) {   // ODA (:163-180)
+      d2d(c, T1.d(), D.d(), nn);
+      axpby_f64(c.s, nn, -1.0, Dtil.d(), 1.0, T1.d());            // dD = D - D~
+      transpose_sq(c, n, T1.d(), 1.0, 0.0, T2.d());               // tr(A dD) = sum A .* dD^T
+      const double sv = dotd(c, nn, Ftil.d(), T2.d(), 10);
+      const double cv = dotd(c, nn, F.d(), T2.d(), 11) - sv;
+      const double lam = (cv <= -sv / (2.0 * cv)) ? 1.0 : -sv / (2.0 * cv);
+      axpby_f64(c.s, nn, lam, F.d(), 1.0 - lam, Ftil.d());
+      axpby_f64(c.s, nn, lam, D.d(), 1.0 - lam, Dtil.d());
+      d2d(c, F.d(), Ftil.d(), nn);
+    } else if (dm && ite > op.diis_start) {                       // DIIS (:183-186)
+      dm->extrapolate(F.d(), false);
+    }
+    d2d(c, T1.d(), D.d(), nn);                                    // Drms (:189-190)
+    axpby_f64(c.s, nn, -1.0, Dold.d(), 1.0, T1.d());
+    Drms = std::sqrt(dotd(c, nn, T1.d(), T1.d(), 12) / (double)nn);
+    dE = Enew - E;
+    E = Enew;
+    d2d(c, Dold.d(), D.d(), nn);
+    ++ite;
+    if (std::fabs(dE) < op.scf_e_conv && Drms < op.scf_max_rms && ite > 5) {   // :200-203
+      converged = true;
+      break;
+    }
+  }
+  FH_CHECK(std::isfinite(E), -5, "fh_rhf: non-finite energy");
+  if (eps_out) d2h(c, eps_out, w.d(), n);
+  if (C_out) d2h(c, C_out, Cm.d(), nn);
+  res.energy = E;
+  res.e_conv = dE;
+  res.d_rms = Drms;
+  res.niter = ite - 1;
+  res.converged = converged ? 1 : 0;
+}
+
 static void solve(Ctx& c, const fh_cc_opts& op, fh_cc_result& res) {
   prepare(c);
   FH_CHECK(c.have_amp, -6, "no starting amplitudes");
@@ -1255,6 +1418,47 @@ int fh_set_df_ao(fh_ctx* x, int nbf, int naux, int ndocc, int drop_occ, int drop
     FH_HIP(hipStreamSynchronize(c.s));
     flush_timers(c);
     c.have_df = true;
+  });
+}
+
+void fh_default_scf_opts(fh_scf_opts* p) {
+  if (!p) return;
+  p->scf_max_iter = 50;
+  p->scf_e_conv = 1e-10;
+  p->scf_max_rms = 1e-9;
+  p->diis = 1;
+  p->ndiis = 8;
+  p->diis_start = 3;
+  p->oda = 1;
+  p->oda_cutoff = 1e-1;
+  p->oda_shutoff = 20;
+  p->guess = 0;
+}
+
+int fh_rhf(fh_ctx* x, const fh_scf_opts* opts, int nbf, int naux, int ndocc, double Vnuc, const double* S,
+           const double* H, const double* bAO, double* eps, double* C, fh_scf_result* res) {
+  return guarded([&] {
+    FH_CHECK(x != nullptr && opts != nullptr && res != nullptr, -1, "fh_rhf: null argument");
+    FH_CHECK(S && H && bAO, -1, "fh_rhf: null array");
+    FH_CHECK(nbf >= 1 && naux >= 1 && ndocc >= 1 && ndocc <= nbf, -1, "fh_rhf: invalid dimensions");
+    FH_CHECK((double)naux * nbf * nbf < 9.0e18 / 8, -1, "fh_rhf: AO tensor too large");
+    const fh_scf_opts& op = *opts;
+    FH_CHECK(op.scf_max_iter >= 0 && (!op.diis || (op.ndiis >= 2 && op.ndiis <= 8)) && (op.guess == 0 || op.guess == 1),
+             -1, "fh_rhf: invalid options (ndiis 2..8, guess 0 = gwh or 1 = core)");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    const int64_t nn = (int64_t)nbf * nbf;
+    Buf dS = dalloc(c, nn), dH = dalloc(c, nn), db = dalloc(c, (int64_t)naux * nn);
+    h2d(c, dS.d(), S, nn);
+    h2d(c, dH.d(), H, nn);
+    h2d(c, db.d(), bAO, (int64_t)naux * nn);
+    *res = fh_scf_result{};
+    {
+      Timer t(c, "scf");
+      rhf(c, op, nbf, naux, ndocc, Vnuc, dS.d(), dH.d(), db.d(), eps, C, *res);
+    }
+    FH_HIP(hipStreamSynchronize(c.s));
+    flush_timers(c);
   });
 }
 

@@ -191,6 +191,10 @@ int64_t mp2_pairs_blocks(int v, int nj);
 void mp2_pairs(hipStream_t s, int o, int v, int j0, int nj, const double* X, const double* eo, const double* ev,
                double* partials);
 
+// SCF: out = U diag(w)^-1/2 (n x n, column-major); GWH guess Fock matrix (RHFHelper.jl:44-51)
+void scale_cols_rsqrt(hipStream_t s, int n, const double* U, const double* w, double* out);
+void gwh_fock(hipStream_t s, int n, const double* S, const double* H, double* F);
+
 // (T) energy for one (i,j) pair and k in [k0, k0+nk): W[a + v*b + v^2*c + v^3*(k-k0)] holds the
 // W intermediate of ijk.jl:108-115; one FP64 partial per workgroup.
 // orbits: device list of cube-block triples (A>=B>=C), 3 ints each (triples_orbits)

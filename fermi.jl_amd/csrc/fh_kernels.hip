@@ -619,4 +619,29 @@ template void triples_energy<double>(hipStream_t, int, int, int, int, int, int, 
 template void triples_energy<float>(hipStream_t, int, int, int, int, int, int, const float*, const int*, const double*,
                                     const double*, const double*, const double*, double*, int64_t*);
 
+// ------------------------------------------------------------------ SCF helpers (RHFHelper.jl:7-59)
+// out[i + n k] = U[i + n k] / sqrt(w[k])  (U diag(w)^-1/2, for S^-1/2 = U diag(w)^-1/2 U^T)
+__global__ void k_scale_cols_rsqrt(int n, const double* __restrict__ U, const double* __restrict__ w,
+                                   double* __restrict__ out) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256)
+    out[idx] = U[idx] / sqrt(w[idx / n]);
+}
+void scale_cols_rsqrt(hipStream_t s, int n, const double* U, const double* w, double* out) {
+  hipLaunchKernelGGL(k_scale_cols_rsqrt, dim3(grid_for((int64_t)n * n)), dim3(256), 0, s, n, U, w, out);
+  FH_HIP(hipGetLastError());
+}
+// generalised Wolfsberg-Helmholz Fock guess: F_ii = H_ii, F_ij = 0.875 S_ij (H_ii + H_jj)
+__global__ void k_gwh_fock(int n, const double* __restrict__ S, const double* __restrict__ H, double* __restrict__ F) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    F[idx] = i == j ? H[idx] : 0.875 * S[idx] * (H[(int64_t)i * (n + 1)] + H[(int64_t)j * (n + 1)]);
+  }
+}
+void gwh_fock(hipStream_t s, int n, const double* S, const double* H, double* F) {
+  hipLaunchKernelGGL(k_gwh_fock, dim3(grid_for((int64_t)n * n)), dim3(256), 0, s, n, S, H, F);
+  FH_HIP(hipGetLastError());
+}
+
 }  // namespace fh

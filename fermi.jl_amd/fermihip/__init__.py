@@ -9,6 +9,7 @@ C ABI in ``include/fermihip.h``:
   * ``RCCSDpT(ccsd, moints, alg=ijkhip())``         -> (T) correction (ijk.jl:20-150)
   * ``RMP2(moints, alg=RMP2hip())``                 -> DF-MP2 energy (RMP2a.jl:56-144); only
     ``BOV`` and the orbital energies are needed (``BOO``/``BVV`` may be None)
+  * ``RHF(S, H, b, ndocc, Vnuc, alg=RHFhip())``      -> DF-RHF SCF (RHFa.jl:14-223)
 
 ``moints`` is a :class:`MOIntegrals`, the subset of Fermi's ``IntegralHelper`` cache the path
 reads: ``BOO[Q,o,o]``, ``BOV[Q,o,v]``, ``BVV[Q,v,v]``, ``Fii[o]``, ``Faa[v]`` and the
@@ -26,13 +27,14 @@ import numpy as np
 
 __all__ = [
     "FermiException", "MOIntegrals", "Options", "RCCSDhip", "ijkhip", "RCCSD", "RCCSDpT", "Context",
-    "lib_path", "load_library", "EXPORTED_SYMBOLS", "Mlp", "PRECISIONS", "RMP2", "RMP2hip",
+    "lib_path", "load_library", "EXPORTED_SYMBOLS", "Mlp", "PRECISIONS", "RMP2", "RMP2hip", "RHF", "RHFhip",
+    "ScfOptions",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXPORTED_SYMBOLS = [
     "fh_default_opts", "fh_create", "fh_set_df", "fh_prepare", "fh_t2_guess", "fh_ccsd_energy",
-    "fh_ccsd_residual", "fh_ccsd_solve", "fh_mp2_energy", "fh_set_df_ao", "fh_get_df", "fh_triples", "fh_stats", "fh_last_error", "fh_destroy",
+    "fh_ccsd_residual", "fh_ccsd_solve", "fh_mp2_energy", "fh_set_df_ao", "fh_get_df", "fh_default_scf_opts", "fh_rhf", "fh_triples", "fh_stats", "fh_last_error", "fh_destroy",
     "fh_mlp_create", "fh_mlp_num_params", "fh_mlp_get_params", "fh_mlp_set_params", "fh_mlp_forward",
     "fh_mlp_loss_grad", "fh_mlp_train", "fh_mlp_destroy", "fh_set_t2_model",
 ]
@@ -59,6 +61,32 @@ class _Result(ctypes.Structure):
         ("t_conv", ctypes.c_double), ("niter", ctypes.c_int), ("converged", ctypes.c_int),
         ("niter_mixed", ctypes.c_int), ("niter_bf3", ctypes.c_int),
     ]
+
+
+class _ScfOpts(ctypes.Structure):
+    _fields_ = [
+        ("scf_max_iter", ctypes.c_int), ("scf_e_conv", ctypes.c_double), ("scf_max_rms", ctypes.c_double),
+        ("diis", ctypes.c_int), ("ndiis", ctypes.c_int), ("diis_start", ctypes.c_int), ("oda", ctypes.c_int),
+        ("oda_cutoff", ctypes.c_double), ("oda_shutoff", ctypes.c_int), ("guess", ctypes.c_int),
+    ]
+
+
+class _ScfResult(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("e_conv", ctypes.c_double), ("d_rms", ctypes.c_double),
+                ("niter", ctypes.c_int), ("converged", ctypes.c_int)]
+
+
+def ScfOptions(**kw):
+    """SCF options with Fermi's key names and defaults (Options.jl:55-89); scf_guess "gwh" | "core"."""
+    o = {"scf_max_iter": 50, "scf_e_conv": 1e-10, "scf_max_rms": 1e-9, "diis": True, "ndiis": 8, "diis_start": 3,
+         "oda": True, "oda_cutoff": 1e-1, "oda_shutoff": 20, "scf_guess": "gwh"}
+    for k in kw:
+        if k not in o:
+            raise FermiException(f"unknown SCF option {k}")
+    o.update(kw)
+    if o["scf_guess"] not in ("gwh", "core"):
+        raise FermiException(f"scf_guess {o['scf_guess']} is not supported")
+    return o
 
 
 def lib_path():
@@ -91,6 +119,10 @@ def load_library():
     L.fh_mp2_energy.argtypes = [ctypes.c_void_p, dp]
     L.fh_set_df_ao.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [dp, dp, dp]
     L.fh_get_df.argtypes = [ctypes.c_void_p, dp, dp, dp]
+    L.fh_default_scf_opts.argtypes = [ctypes.POINTER(_ScfOpts)]
+    L.fh_default_scf_opts.restype = None
+    L.fh_rhf.argtypes = [ctypes.c_void_p, ctypes.POINTER(_ScfOpts), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                         ctypes.c_double, dp, dp, dp, dp, dp, ctypes.POINTER(_ScfResult)]
     L.fh_stats.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     L.fh_last_error.argtypes = []
     L.fh_last_error.restype = ctypes.c_char_p
@@ -237,6 +269,27 @@ class Context:
                                         *[_ptr(x) for x in keep]))
         self.o, self.v, self.naux = ndocc - drop_occ, n - drop_vir - ndocc, Q
 
+    def rhf(self, S, H, b, ndocc, Vnuc=0.0, options=None):
+        """DF-RHF SCF (fh_rhf): returns (result dict, eps[nbf], C[nbf,nbf])."""
+        Q, n, n2 = b.shape
+        if n2 != n or S.shape != (n, n) or H.shape != (n, n):
+            raise FermiException("inconsistent S / H / b shapes")
+        o = ScfOptions() if options is None else options
+        op = _ScfOpts()
+        self.L.fh_default_scf_opts(ctypes.byref(op))
+        for k in ("scf_max_iter", "scf_e_conv", "scf_max_rms", "ndiis", "diis_start", "oda_cutoff", "oda_shutoff"):
+            setattr(op, k, o[k])
+        op.diis, op.oda, op.guess = int(bool(o["diis"])), int(bool(o["oda"])), 0 if o["scf_guess"] == "gwh" else 1
+        keep = [_colmajor(x) for x in (S, H, b)]
+        eps = np.zeros(n)
+        C = np.zeros((n, n), order="F")
+        res = _ScfResult()
+        self._check(self.L.fh_rhf(self.h, ctypes.byref(op), n, Q, int(ndocc), float(Vnuc), *[_ptr(x) for x in keep],
+                                  _ptr(eps), _ptr(C), ctypes.byref(res)))
+        r = {"energy": res.energy, "e_conv": res.e_conv, "d_rms": res.d_rms, "niter": res.niter,
+             "converged": bool(res.converged)}
+        return r, eps, C
+
     def get_df(self):
         """The MO DF tensors held by the context: (BOO[Q,o,o], BOV[Q,o,v], BVV[Q,v,v])."""
         Q, o, v = self.naux, self.o, self.v
@@ -375,6 +428,38 @@ def _ctx_for(moints, device=0):
         moints.context = Context(device)
         moints.context.set_df(moints.Fii, moints.Faa, moints.BOO, moints.BOV, moints.BVV)
     return moints.context
+
+
+class RHFhip:
+    """RHF algorithm singleton for the MI355X backend (the role of RHFa, RHF.jl:112)."""
+
+
+@dataclass
+class RHFResult:
+    """Fields of Fermi's RHF (RHF.jl:69-77), with the orbitals as eps / C."""
+    energy: float
+    ndocc: int
+    nvir: int
+    eps: np.ndarray
+    C: np.ndarray
+    e_conv: float
+    d_conv: float
+    converged: bool
+    niter: int
+
+
+def RHF(S, H, b, ndocc, Vnuc=0.0, alg=None, options=None, device=0):
+    """RHF(ints, ::RHFa) with DF integrals (RHFa.jl:14-223): S = ints["S"], H = ints["T"] + ints["V"],
+    b = ints["ERI"] (fitted AO tensor b[Q,mu,nu]), ndocc = Nalpha, Vnuc the nuclear repulsion."""
+    if alg is not None and not isinstance(alg, RHFhip):
+        raise FermiException(f"invalid algorithm {type(alg).__name__} for the MI355X RHF backend")
+    ctx = Context(device)
+    try:
+        r, eps, C = ctx.rhf(S, H, b, ndocc, Vnuc, options)
+    finally:
+        ctx.close()
+    return RHFResult(r["energy"], int(ndocc), S.shape[0] - int(ndocc), eps, C, r["e_conv"], r["d_rms"],
+                     r["converged"], r["niter"])
 
 
 class RMP2hip:

@@ -33,7 +33,10 @@ def build(verbose=False):
     if failed:
         raise RuntimeError(f"hipcc failed on {', '.join(failed)}")
     tmp = LIB + ".tmp"
-    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(hipcc))), "lib")
+    # rocSOLVER (symmetric eigensolver of the SCF) and the rocBLAS handle it runs on
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + [
+        "-L" + rocm_lib, "-Wl,-rpath," + rocm_lib, "-lrocsolver", "-lrocblas"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

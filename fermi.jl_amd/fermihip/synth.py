@@ -19,10 +19,13 @@ over from Fermi's IntegralHelper), so workloads are generated from a seed:
 * :func:`triples_config` -- the (T)-only synthetic tensors: eps_occ ~ U(-1.5,-0.3),
   eps_vir ~ U(0.1,3.0), B[Q,p,q] ~ N(0,1) exp(-Q/300) (symmetrised in p,q as real
   orbitals require), T2 = MP2 amplitudes from those denominators, T1 = 0 or N(0,1e-3).
+* :func:`scf_like` -- an AO-basis closed-shell SCF problem for the DF-RHF path: overlap S (unit
+  diagonal, positive definite), core Hamiltonian H with a spread diagonal, fitted AO tensor
+  b[Q,mu,nu] symmetric in (mu, nu) and scaled so that J and K do not grow with the size.
 """
 import numpy as np
 
-__all__ = ["molecular_like", "water_cluster_like", "pair_distances", "triples_config", "mp2_amplitudes", "SHAPES",
+__all__ = ["molecular_like", "water_cluster_like", "pair_distances", "triples_config", "mp2_amplitudes", "scf_like", "SHAPES",
            "WATERS"]
 
 # (o, v, naux) of the benchmark configurations (cc-pVDZ / cc-pVDZ-RI water clusters, frozen core)
@@ -149,3 +152,20 @@ def pair_distances(sites_o, pos):
     """Occupied pair distances d[i,j] (bohr) between the waters of orbitals i and j."""
     p = pos[sites_o]
     return np.linalg.norm(p[:, None, :] - p[None, :, :], axis=-1)
+
+
+def scf_like(nbf, naux, seed=0):
+    """(S, H, b) of a synthetic closed-shell SCF problem."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((nbf, nbf))
+    S = np.eye(nbf) + 0.05 * (A + A.T)
+    lo = np.linalg.eigvalsh(S).min()
+    if lo < 0.3:
+        S = S + (abs(lo) + 0.3) * np.eye(nbf)
+    d = np.sqrt(np.diag(S))
+    S = S / d[:, None] / d[None, :]
+    B = rng.standard_normal((nbf, nbf))
+    H = np.diag(np.linspace(-5.0, 2.0, nbf)) + 0.1 * (B + B.T)
+    G = rng.standard_normal((naux, nbf, nbf))
+    b = 0.06 * np.sqrt(420.0 / (nbf * naux)) * (G + np.transpose(G, (0, 2, 1)))
+    return S, H, b

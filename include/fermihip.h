@@ -95,6 +95,40 @@ int fh_set_df(fh_ctx* ctx, int o, int v, int naux, const double* eps_o, const do
 int fh_set_df_ao(fh_ctx* ctx, int nbf, int naux, int ndocc, int drop_occ, int drop_vir, const double* eps,
                  const double* C, const double* bAO);
 
+/* DF-RHF SCF options: the keys of Fermi's Options.jl:55-89 (fh_default_scf_opts gives their defaults). */
+typedef struct fh_scf_opts {
+  int scf_max_iter;    /* 50 */
+  double scf_e_conv;   /* 1e-10 */
+  double scf_max_rms;  /* 1e-9 */
+  int diis;            /* 1 */
+  int ndiis;           /* 8 (2..8 here) */
+  int diis_start;      /* 3 */
+  int oda;             /* 1 */
+  double oda_cutoff;   /* 1e-1 */
+  int oda_shutoff;     /* 20 */
+  int guess;           /* scf_guess: 0 = "gwh" (default), 1 = "core" */
+} fh_scf_opts;
+
+/* Scalar fields of Fermi's RHF wave function (RHF.jl): total energy, last Delta E and D rms. */
+typedef struct fh_scf_result {
+  double energy;
+  double e_conv;
+  double d_rms;
+  int niter;
+  int converged;
+} fh_scf_result;
+
+void fh_default_scf_opts(fh_scf_opts* opts);
+
+/* RHF(ints, ::RHFa) with density-fitted integrals (RHFa.jl:14-223): the GWH or core guess
+ * (RHFHelper.jl:7-59), Lambda = S^-1/2, then the iteration with the DF build_fock!
+ * (RHFHelper.jl:88-93), ODA damping and DIIS.  S = ints["S"], H = ints["T"] + ints["V"] [nbf,nbf],
+ * bAO[Q,mu,nu] = ints["ERI"] (the fitted AO tensor), ndocc = Nalpha, Vnuc the nuclear repulsion.
+ * Outputs eps[nbf] and C[nbf,nbf] of the last diagonalisation (either may be NULL) and res.
+ * Non-convergence is not an error (res->converged = 0), as in RHFa.jl:213-218. */
+int fh_rhf(fh_ctx* ctx, const fh_scf_opts* opts, int nbf, int naux, int ndocc, double Vnuc, const double* S,
+           const double* H, const double* bAO, double* eps, double* C, fh_scf_result* res);
+
 /* Copy the context's MO DF tensors back to the host (the moints cache entries "BOO", "BOV", "BVV"
  * that DFERI.jl:29,50,68 store); any pointer may be NULL. */
 int fh_get_df(fh_ctx* ctx, double* Boo, double* Bov, double* Bvv);

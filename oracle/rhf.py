@@ -1,0 +1,149 @@
+"""CPU oracle (test infrastructure only): FP64 restatement of Fermi.jl's RHF SCF ``RHFa`` with
+density-fitted integrals: the GWH and core guesses (RHFHelper.jl:7-59), ``RHFEnergy``
+(RHFHelper.jl:61-68), the DF ``build_fock!`` (RHFHelper.jl:88-93) and the iteration with ODA
+damping and DIIS (RHFa.jl:67-223, DIIS.jl).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg may import this
+module, and only as the checker.  The product path (``libfermihip.so``) never calls it.
+
+Arrays are indexed like the Julia arrays: ``b[Q,mu,nu]``, ``S[mu,nu]``, ``C[mu,p]``.
+
+Pins (``tests/test_oracle_rhf.py``):
+  * the DF Fock build equals the conventional one (RHFHelper.jl:68-73) on the four-index tensor
+    (mu nu|la si) = sum_Q b b;
+  * a symmetric two-function, one-pair system, whose orbital is fixed by symmetry: the closed form
+    E = 2 h_gg + (gg|gg) + Vnuc;
+  * at convergence: the Brillouin condition FDS = SDF, C^T S C = 1, eps = diag(C^T F C), and the
+    energy rises under small occupied-virtual rotations (variational minimum).
+"""
+import numpy as np
+
+__all__ = ["Options", "inv_sqrt", "core_guess", "gwh_guess", "rhf_energy", "build_fock_df", "build_fock_conv",
+           "rhf"]
+
+
+def Options(**overrides):
+    """The SCF keys of Fermi's Options.jl:55-89 with their defaults."""
+    o = {"scf_max_iter": 50, "scf_max_rms": 1e-9, "scf_e_conv": 1e-10, "oda": True, "oda_cutoff": 1e-1,
+         "oda_shutoff": 20, "scf_guess": "gwh", "diis": True, "ndiis": 8, "diis_start": 3}
+    o.update(overrides)
+    return o
+
+
+def _eigh(A):
+    """eigen(Symmetric(A), sortby=x->x): ascending eigenvalues."""
+    return np.linalg.eigh(0.5 * (A + A.T))
+
+
+def inv_sqrt(S):
+    """S^(-1/2) of a symmetric positive definite matrix (RHFHelper.jl:9,36)."""
+    d, U = _eigh(S)
+    return (U / np.sqrt(d)) @ U.T
+
+
+def core_guess(S, H):
+    """RHF_core_guess (RHFHelper.jl:7-21): Ft = L H L', C = L Ct."""
+    L = inv_sqrt(S)
+    _, Ct = _eigh(L @ H @ L.T)
+    return L @ Ct, L
+
+
+def gwh_guess(S, H):
+    """RHF_gwh_guess (RHFHelper.jl:28-59): F_ii = H_ii, F_ij = 0.875 S_ij (H_ii + H_jj)."""
+    L = inv_sqrt(S)
+    dH = np.diag(H)
+    F = 0.875 * S * (dH[:, None] + dH[None, :])
+    np.fill_diagonal(F, dH)
+    _, Ct = _eigh(L.T @ F @ L)
+    return L @ Ct, L
+
+
+def rhf_energy(D, H, F):
+    """RHFEnergy = sum(D .* (H .+ F)) (RHFHelper.jl:61-68)."""
+    return float(np.sum(D * (H + F)))
+
+
+def build_fock_df(H, D, b):
+    """F = H + 2 D[r,s] b[Q,m,n] b[Q,r,s] - D[r,s] b[Q,m,r] b[Q,n,s] (RHFHelper.jl:88-93)."""
+    X = np.tensordot(b, D, axes=([1, 2], [0, 1]))                      # X[Q] = b[Q,r,s] D[r,s]
+    J = np.tensordot(X, b, axes=(0, 0))                                 # J[m,n]
+    bD = np.tensordot(b, D, axes=(2, 0))                                # bD[Q,m,s] = b[Q,m,r] D[r,s]
+    K = np.tensordot(bD, b, axes=([0, 2], [0, 2]))                      # K[m,n] = bD[Q,m,s] b[Q,n,s]
+    return H + 2.0 * J - K
+
+
+def build_fock_conv(H, D, eri):
+    """Conventional build_fock! (RHFHelper.jl:68-73): F = H + 2 D[r,s] (mn|rs) - D[r,s] (mr|ns)."""
+    return H + 2.0 * np.einsum("rs,mnrs->mn", D, eri) - np.einsum("rs,mrns->mn", D, eri)
+
+
+class _DIIS:
+    """Fermi.DIIS.DIISManager (DIIS.jl:21-96) with add_res = false."""
+
+    def __init__(self, size):
+        self.size, self.vecs, self.errs = size, [], []
+
+    def push(self, v, e):
+        if len(self.vecs) + 1 > self.size:
+            idx = int(np.argmax([np.linalg.norm(x) for x in self.errs]))
+            del self.vecs[idx], self.errs[idx]
+        self.vecs.append(v.copy())
+        self.errs.append(e.copy())
+
+    def extrapolate(self):
+        n = len(self.vecs)
+        if n < 2:
+            raise IndexError("DIIS cannot extrapolate from one vector")
+        B = np.ones((n + 1, n + 1))
+        B[n, n] = 0.0
+        for i in range(n):
+            for j in range(n):
+                B[i, j] = np.sum(self.errs[i] * self.errs[j])
+        r = np.zeros(n + 1)
+        r[n] = 1.0
+        c = np.linalg.solve(B, r)
+        return sum(c[i] * self.vecs[i] for i in range(n))
+
+
+def rhf(S, H, b, ndocc, Vnuc=0.0, opts=None):
+    """RHF(ints, C, Lambda, ::RHFa) (RHFa.jl:67-223) with the guess of RHFa.jl:24-31.
+    Returns a dict with energy, eps, C, converged, niter (iterations run), dE, Drms."""
+    op = Options() if opts is None else opts
+    C, L = gwh_guess(S, H) if op["scf_guess"] == "gwh" else core_guess(S, H)
+    dm = _DIIS(op["ndiis"]) if op["diis"] else None
+    Co = C[:, :ndocc]
+    D = Co @ Co.T                                                        # :119
+    D_old = D.copy()
+    eps = np.zeros(C.shape[1])
+    F = build_fock_df(H, D, b)                                           # :125
+    Ft_, Dt_ = F.copy(), D.copy()                                        # F~, D~ (:126-127)
+    N = D.size
+    ite, E, dE, Drms, converged = 1, 0.0, 1.0, 1.0, False
+    while ite <= op["scf_max_iter"]:
+        eps, Ct = _eigh(L.T @ F @ L)                                     # :137-140
+        C = L @ Ct                                                       # :143
+        Co = C[:, :ndocc]
+        D = Co @ Co.T                                                    # :147
+        F = build_fock_df(H, D, b)                                       # :150
+        Enew = rhf_energy(D, H, F) + Vnuc                                # :151-154
+        if dm is not None:
+            dm.push(F, L.T @ (F @ D @ S - S @ D @ F) @ L)                # :157-160
+        if op["oda"] and Drms > op["oda_cutoff"] and ite < op["oda_shutoff"]:
+            dD = D - Dt_                                                 # :168-180
+            s = np.trace(Ft_ @ dD)
+            c = np.trace((F - Ft_) @ dD)
+            lam = 1.0 if c <= -s / (2.0 * c) else -s / (2.0 * c)
+            Ft_ = (1.0 - lam) * Ft_ + lam * F
+            Dt_ = (1.0 - lam) * Dt_ + lam * D
+            F = Ft_.copy()
+        elif dm is not None and ite > op["diis_start"]:
+            F = dm.extrapolate()                                         # :183-186
+        Drms = float(np.sqrt(np.sum((D - D_old) ** 2) / N))              # :189-190
+        dE = Enew - E
+        E = Enew
+        D_old = D.copy()
+        ite += 1
+        if abs(dE) < op["scf_e_conv"] and Drms < op["scf_max_rms"] and ite > 5:   # :200-203
+            converged = True
+            break
+    return {"energy": E, "eps": eps, "C": C, "converged": converged, "niter": ite - 1, "dE": dE, "Drms": Drms}

@@ -26,3 +26,21 @@ def relerr(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def random_scf(nbf, naux, seed):
+    """A closed-shell SCF problem with molecule-like structure: overlap S (unit diagonal, SPD),
+    core Hamiltonian H with spread diagonal, AO DF tensor b[Q,mu,nu] symmetric in (mu, nu)."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((nbf, nbf))
+    S = np.eye(nbf) + 0.05 * (A + A.T)
+    lo = np.linalg.eigvalsh(S).min()
+    if lo < 0.3:
+        S = S + (abs(lo) + 0.3) * np.eye(nbf)
+    d = np.sqrt(np.diag(S))
+    S = S / d[:, None] / d[None, :]
+    B = rng.standard_normal((nbf, nbf))
+    H = np.diag(np.linspace(-5.0, 2.0, nbf)) + 0.1 * (B + B.T)
+    G = rng.standard_normal((naux, nbf, nbf))
+    b = 0.06 * np.sqrt(420.0 / (nbf * naux)) * (G + np.transpose(G, (0, 2, 1)))   # J, K ~ size-independent
+    return S, H, b
