@@ -139,9 +139,10 @@ def main():
     ap.add_argument("--verbose", action="store_true", help="print the CC iteration table")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--phase", default="full", choices=["full", "triples", "copy"],
+    ap.add_argument("--phase", default="full", choices=["full", "triples", "copy", "front-end"],
                     help="full: the benchmark step; triples: integrals + guess + (T) only (profiling runs); "
-                         "copy: one plain 1 GiB device copy (calibration of the HBM byte counters)")
+                         "copy: one plain 1 GiB device copy (calibration of the HBM byte counters); front-end: "
+                         "only the DF-MP2, AO->MO transform and DF-RHF legs (profiling runs)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -193,6 +194,63 @@ def main():
                             mixed_rms_switch=mxr)
     tprec = fermihip.PRECISIONS[args.precision]
 
+    def front_end_legs():
+        """The callers either side of the CC path on this workload's shapes (outside the timed
+        step): DF-MP2 of the context's DF tensors, the AO->MO transform and DF-RHF at the AO shape."""
+        # DF-MP2 energy of the same DF tensors (fh_mp2_energy, FP64): one untimed call, then three
+        e_mp2 = ctx.mp2_energy()
+        b_mp2 = ctx.stats()["phases"]
+        for _ in range(3):
+            e_mp2 = ctx.mp2_energy()
+        p_mp2 = phase_diff(ctx.stats()["phases"], b_mp2).get("df_mp2")
+        df_mp2 = None
+        if p_mp2 and p_mp2["ms"] > 0:
+            tf = p_mp2["flops"] / (p_mp2["ms"] * 1e-3) / 1e12
+            df_mp2 = {"correlation": e_mp2, "ms": p_mp2["ms"] / 3, "tflops": tf,
+                      "frac_fp64_mfma_peak": tf / FP64_MFMA_PEAK_TFLOPS}
+            log(f"bench: DF-MP2 {df_mp2['ms']:.2f} ms, {df_mp2['tflops']:.1f} TFLOP/s, Emp2={e_mp2:.12f}")
+        # AO -> MO DF transform (fh_set_df_ao, FP64) at this workload's orbital shape: one frozen 1s core
+        # per water, nbf = o + v + cores; random fitted AO tensor and orthogonal MO coefficients
+        rng = np.random.default_rng(args.seed + 7)
+        ncore = nw
+        nbf = o + v + ncore
+        b_ao = np.asfortranarray(0.01 * rng.standard_normal((Q, nbf, nbf)))
+        C_ao = np.asfortranarray(np.linalg.qr(rng.standard_normal((nbf, nbf)))[0])
+        eps_ao = np.sort(np.concatenate([eo.min() - 20.0 + np.arange(ncore), eo, ev]))
+        ctx.set_df_ao(eps_ao, C_ao, b_ao, o + ncore, ncore, 0)
+        b_ph = ctx.stats()["phases"]
+        for _ in range(3):
+            ctx.set_df_ao(eps_ao, C_ao, b_ao, o + ncore, ncore, 0)
+        p_ao = phase_diff(ctx.stats()["phases"], b_ph).get("ao_to_mo")
+        df_ao = None
+        if p_ao and p_ao["ms"] > 0:
+            tf = p_ao["flops"] / (p_ao["ms"] * 1e-3) / 1e12
+            df_ao = {"nbf": nbf, "ndocc": o + ncore, "drop_occ": ncore, "ms": p_ao["ms"] / 3, "tflops": tf,
+                     "frac_fp64_mfma_peak": tf / FP64_MFMA_PEAK_TFLOPS}
+            log(f"bench: AO->MO DF transform nbf={nbf} {df_ao['ms']:.2f} ms, {tf:.1f} TFLOP/s")
+        del b_ao
+        # DF-RHF SCF (fh_rhf, FP64) at the same AO shape: synthetic S, H, b; one untimed call first
+        S_s, H_s, b_s = synth.scf_like(nbf, Q, args.seed)
+        ctx.rhf(S_s, H_s, b_s, o + ncore)
+        b_ph = ctx.stats()["phases"]
+        t0 = time.perf_counter()
+        r_scf, _, _ = ctx.rhf(S_s, H_s, b_s, o + ncore)
+        t_scf = time.perf_counter() - t0
+        ph_s = phase_diff(ctx.stats()["phases"], b_ph)
+        df_rhf = None
+        if ph_s.get("scf") and ph_s.get("scf_fock"):
+            fk, dg = ph_s["scf_fock"], ph_s["scf_diag"]
+            df_rhf = {"nbf": nbf, "naux": Q, "ndocc": o + ncore, "niter": r_scf["niter"],
+                      "converged": r_scf["converged"], "energy": r_scf["energy"], "device_ms": ph_s["scf"]["ms"], "pcie_inclusive_s": t_scf,
+                      "fock_ms_per_call": fk["ms"] / max(1, fk["calls"]),
+                      "fock_tflops": fk["flops"] / (fk["ms"] * 1e-3) / 1e12,
+                      "diag_ms_per_call": dg["ms"] / max(1, dg["calls"])}
+            log(f"bench: DF-RHF nbf={nbf} {r_scf['niter']} iterations, {ph_s['scf']['ms']:.1f} ms on the device "
+                f"(Fock {df_rhf['fock_ms_per_call']:.2f} ms/call at {df_rhf['fock_tflops']:.1f} TFLOP/s, "
+                f"diagonalisation {df_rhf['diag_ms_per_call']:.2f} ms/call)")
+        del b_s
+        return df_mp2, df_ao, df_rhf
+
     def step():
         t0 = time.perf_counter()
         ctx.set_df(eo, ev, Boo, Bov, Bvv)      # host -> HBM (untimed part of the step)
@@ -209,6 +267,14 @@ def main():
         Et = ctx.triples(prec_mode=tprec)
         t2 = time.perf_counter()
         return t2 - t1, t1 - t0, r, Et
+
+    if args.phase == "front-end":
+        ctx.set_df(eo, ev, Boo, Bov, Bvv)
+        df_mp2, df_ao, df_rhf = front_end_legs()
+        print(json.dumps({"phase": "front-end", "config": args.config, "df_mp2": df_mp2, "df_ao_to_mo": df_ao,
+                          "df_rhf": df_rhf}), flush=True)
+        ctx.close()
+        return
 
     if args.phase == "triples":
         for k in range(max(1, args.steps)):
@@ -307,57 +373,7 @@ def main():
                     "error_vs_fp64_Eh": {"ccsd": r.correlation - r64.correlation, "triples": Et - Et64,
                                          "total": (r.correlation + Et) - (r64.correlation + Et64)}}
         fp64_ref["within_1uEh"] = abs(fp64_ref["error_vs_fp64_Eh"]["total"]) <= 1e-6
-    # DF-MP2 energy of the same DF tensors (fh_mp2_energy, FP64): one untimed call, then three
-    e_mp2 = ctx.mp2_energy()
-    b_mp2 = ctx.stats()["phases"]
-    for _ in range(3):
-        e_mp2 = ctx.mp2_energy()
-    p_mp2 = phase_diff(ctx.stats()["phases"], b_mp2).get("df_mp2")
-    df_mp2 = None
-    if p_mp2 and p_mp2["ms"] > 0:
-        df_mp2 = {"correlation": e_mp2, "ms": p_mp2["ms"] / 3, "tflops": p_mp2["flops"] / (p_mp2["ms"] * 1e-3) / 1e12,
-                  "frac_fp64_mfma_peak": p_mp2["flops"] / (p_mp2["ms"] * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS}
-        log(f"bench: DF-MP2 {df_mp2['ms']:.2f} ms, {df_mp2['tflops']:.1f} TFLOP/s, Emp2={e_mp2:.12f}")
-    # AO -> MO DF transform (fh_set_df_ao, FP64) at this workload's orbital shape: one frozen 1s core
-    # per water, nbf = o + v + cores; random fitted AO tensor and orthogonal MO coefficients
-    rng = np.random.default_rng(args.seed + 7)
-    ncore = nw
-    nbf = o + v + ncore
-    b_ao = np.asfortranarray(0.01 * rng.standard_normal((Q, nbf, nbf)))
-    C_ao = np.asfortranarray(np.linalg.qr(rng.standard_normal((nbf, nbf)))[0])
-    eps_ao = np.sort(np.concatenate([eo.min() - 20.0 + np.arange(ncore), eo, ev]))
-    ctx.set_df_ao(eps_ao, C_ao, b_ao, o + ncore, ncore, 0)
-    b_ph = ctx.stats()["phases"]
-    for _ in range(3):
-        ctx.set_df_ao(eps_ao, C_ao, b_ao, o + ncore, ncore, 0)
-    p_ao = phase_diff(ctx.stats()["phases"], b_ph).get("ao_to_mo")
-    df_ao = None
-    if p_ao and p_ao["ms"] > 0:
-        tf = p_ao["flops"] / (p_ao["ms"] * 1e-3) / 1e12
-        df_ao = {"nbf": nbf, "ndocc": o + ncore, "drop_occ": ncore, "ms": p_ao["ms"] / 3, "tflops": tf,
-                 "frac_fp64_mfma_peak": tf / FP64_MFMA_PEAK_TFLOPS}
-        log(f"bench: AO->MO DF transform nbf={nbf} {df_ao['ms']:.2f} ms, {tf:.1f} TFLOP/s")
-    del b_ao
-    # DF-RHF SCF (fh_rhf, FP64) at the same AO shape: synthetic S, H, b; one untimed call first
-    S_s, H_s, b_s = synth.scf_like(nbf, Q, args.seed)
-    ctx.rhf(S_s, H_s, b_s, o + ncore)
-    b_ph = ctx.stats()["phases"]
-    t0 = time.perf_counter()
-    r_scf, _, _ = ctx.rhf(S_s, H_s, b_s, o + ncore)
-    t_scf = time.perf_counter() - t0
-    ph_s = phase_diff(ctx.stats()["phases"], b_ph)
-    df_rhf = None
-    if ph_s.get("scf") and ph_s.get("scf_fock"):
-        fk, dg = ph_s["scf_fock"], ph_s["scf_diag"]
-        df_rhf = {"nbf": nbf, "naux": Q, "ndocc": o + ncore, "niter": r_scf["niter"], "converged": r_scf["converged"],
-                  "energy": r_scf["energy"], "device_ms": ph_s["scf"]["ms"], "pcie_inclusive_s": t_scf,
-                  "fock_ms_per_call": fk["ms"] / max(1, fk["calls"]),
-                  "fock_tflops": fk["flops"] / (fk["ms"] * 1e-3) / 1e12,
-                  "diag_ms_per_call": dg["ms"] / max(1, dg["calls"])}
-        log(f"bench: DF-RHF nbf={nbf} {r_scf['niter']} iterations, {ph_s['scf']['ms']:.1f} ms on the device "
-            f"(Fock {df_rhf['fock_ms_per_call']:.2f} ms/call at {df_rhf['fock_tflops']:.1f} TFLOP/s, diagonalisation "
-            f"{df_rhf['diag_ms_per_call']:.2f} ms/call)")
-    del b_s
+    df_mp2, df_ao, df_rhf = front_end_legs()
     out = {
         "metric": "DF-RCCSD(T) time-to-solution (s) + MFMA TFLOP/s at <=1 uEh vs FP64 GPU path",
         "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
