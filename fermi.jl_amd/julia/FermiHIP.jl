@@ -9,7 +9,8 @@ Selection: `@set cc_alg 2` picks `RCCSDhip` through `get_rccsd_alg(Val(2))` (the
 Fermi's RCCSD.jl:76-88); `@set pt_alg 4` picks `ijkhip` through the `get_rpt_alg` method
 below (PerturbativeTriples.jl:3-11 keeps a fixed vector, so this module redefines it);
 `@set mp2_alg 3` picks `RMP2hip` through `get_rmp2_alg(Val(3))` (RMP2.jl:60-77), whose DF energy
-`RMP2_energy` (the method RMP2a.jl:80 calls) runs on the device.
+`RMP2_energy` (the method RMP2a.jl:80 calls) runs on the device; `@set rhf_alg 3` picks `RHFhip`
+through `get_rhf_alg(Val(3))` (RHF.jl:97-118), the DF-RHF SCF of RHFa.jl:14-223 on the device.
 """
 module FermiHIP
 
@@ -20,15 +21,19 @@ using Fermi.CoupledCluster: RCCSDAlgorithm, RpTAlgorithm, ijk, ijk2, abc
 import Fermi.CoupledCluster: RCCSD, RCCSDpT, get_rccsd_alg, get_rpt_alg
 using Fermi.MollerPlesset: RMP2Algorithm
 import Fermi.MollerPlesset: RMP2_energy, get_rmp2_alg
+using Fermi.HartreeFock: RHFAlgorithm
+import Fermi.HartreeFock: RHF, get_rhf_alg
 
 const libfermihip = get(ENV, "FERMIHIP_LIB", joinpath(@__DIR__, "..", "fermihip", "libfermihip.so"))
 
 struct RCCSDhip <: RCCSDAlgorithm end
 struct ijkhip <: RpTAlgorithm end
 struct RMP2hip <: RMP2Algorithm end
+struct RHFhip <: RHFAlgorithm end
 
 get_rccsd_alg(::Val{2}) = RCCSDhip()
 get_rmp2_alg(::Val{3}) = RMP2hip()
+get_rhf_alg(::Val{3}) = RHFhip()
 function get_rpt_alg()
     N = Options.get("pt_alg")
     N == 4 && return ijkhip()
@@ -213,6 +218,61 @@ function RCCSDpT(alg::ijkhip)
     ccsd, moints = RCCSD()
     Options.set("return_ints", val)
     return RCCSDpT(ccsd, moints, alg)
+end
+
+# --- DF-RHF SCF (RHFa.jl:14-223) through fh_rhf
+struct SCFOpts
+    scf_max_iter::Cint
+    scf_e_conv::Cdouble
+    scf_max_rms::Cdouble
+    diis::Cint
+    ndiis::Cint
+    diis_start::Cint
+    oda::Cint
+    oda_cutoff::Cdouble
+    oda_shutoff::Cint
+    guess::Cint
+end
+
+struct SCFResult
+    energy::Cdouble
+    e_conv::Cdouble
+    d_rms::Cdouble
+    niter::Cint
+    converged::Cint
+end
+
+function RHF(ints::IntegralHelper{Float64,<:AbstractDFERI,Fermi.Orbitals.AtomicOrbitals}, alg::RHFhip)
+    molecule = ints.molecule
+    ndocc = try
+        Int((molecule.Nα + molecule.Nβ) / 2)
+    catch InexactError
+        throw(Fermi.Options.FermiException("Invalid number of electrons $(molecule.Nα + molecule.Nβ) for RHF method."))
+    end
+    guess = Options.get("scf_guess")
+    guess in ("gwh", "core") || throw(Fermi.Options.FermiException("scf_guess $guess is not supported by RHFhip"))
+    S = Array{Float64}(ints["S"])
+    H = Array{Float64}(ints["T"] .+ ints["V"])
+    b = Array{Float64}(ints["ERI"])
+    Q, nbf, _ = size(b)
+    Vnuc = Fermi.Molecules.nuclear_repulsion(molecule.atoms)
+    opts = Ref(SCFOpts(Options.get("scf_max_iter"), Options.get("scf_e_conv"), Options.get("scf_max_rms"),
+                       Options.get("diis") ? 1 : 0, Options.get("ndiis"), Options.get("diis_start"),
+                       Options.get("oda") ? 1 : 0, Options.get("oda_cutoff"), Options.get("oda_shutoff"),
+                       guess == "gwh" ? 0 : 1))
+    eps = zeros(Float64, nbf)
+    C = zeros(Float64, nbf, nbf)
+    res = Ref{SCFResult}()
+    ctx = Context(0)
+    GC.@preserve S H b eps C begin
+        check(ccall((:fh_rhf, libfermihip), Cint,
+                    (Ptr{Cvoid}, Ref{SCFOpts}, Cint, Cint, Cint, Cdouble, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                     Ptr{Float64}, Ptr{Float64}, Ref{SCFResult}),
+                    ctx.h, opts, nbf, Q, ndocc, Vnuc, S, H, b, eps, C, res))
+    end
+    r = res[]
+    orbitals = Fermi.Orbitals.RHFOrbitals(molecule, ints.basis, eps, r.energy, C)
+    return Fermi.HartreeFock.RHF(molecule, r.energy, ndocc, nbf - ndocc, orbitals, r.e_conv, r.d_rms)
 end
 
 end # module
