@@ -249,6 +249,19 @@ def main():
                 f"(Fock {df_rhf['fock_ms_per_call']:.2f} ms/call at {df_rhf['fock_tflops']:.1f} TFLOP/s, "
                 f"diagonalisation {df_rhf['diag_ms_per_call']:.2f} ms/call)")
         del b_s
+        # DF fit (fh_df_fit, FP64): J^-1/2 of a synthetic positive definite Coulomb metric [naux, naux]
+        # and b = Pqp J^-1/2 at the AO shape; one untimed call first
+        A_f = rng.standard_normal((Q, Q))
+        J_f = A_f @ A_f.T / Q + 0.5 * np.eye(Q)
+        P_f = np.asfortranarray(rng.standard_normal((nbf, nbf, Q)))
+        ctx.df_fit(J_f, P_f)
+        b_ph = ctx.stats()["phases"]
+        ctx.df_fit(J_f, P_f)
+        p_fit = phase_diff(ctx.stats()["phases"], b_ph).get("df_fit")
+        if df_rhf is not None and p_fit and p_fit["ms"] > 0:
+            df_rhf["df_fit_ms"] = p_fit["ms"]
+            log(f"bench: DF fit naux={Q} nbf={nbf} {p_fit['ms']:.2f} ms on the device")
+        del P_f
         return df_mp2, df_ao, df_rhf
 
     def step():

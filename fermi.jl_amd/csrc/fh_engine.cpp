@@ -1462,6 +1462,39 @@ int fh_rhf(fh_ctx* x, const fh_scf_opts* opts, int nbf, int naux, int ndocc, dou
   });
 }
 
+// The DF fit of compute_ERI! (AtomicIntegrals.jl:31-41): Jh = J^-1/2 = U diag(d)^-1/2 U^T from the
+// eigenpairs of the Coulomb metric, then b[Q,(mu,nu)] = sum_P Jh[P,Q] Pqp[(mu,nu),P] as one FP64 MFMA
+// GEMM (M = naux, N = nbf^2, K = naux) that writes b with Q fastest.
+int fh_df_fit(fh_ctx* x, int nbf, int naux, const double* J, const double* Pqp, double* b) {
+  return guarded([&] {
+    FH_CHECK(x != nullptr && J && Pqp && b, -1, "fh_df_fit: null argument");
+    FH_CHECK(nbf >= 1 && naux >= 1, -1, "fh_df_fit: invalid dimensions");
+    FH_CHECK((double)naux * nbf * nbf < 9.0e18 / 8, -1, "fh_df_fit: AO tensor too large");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    const int64_t Q = naux, nn = (int64_t)nbf * nbf;
+    Buf dJ = dalloc(c, Q * Q), U = dalloc(c, Q * Q), Us = dalloc(c, Q * Q), w = dalloc(c, Q), dP = dalloc(c, nn * Q),
+        db = dalloc(c, nn * Q);
+    h2d(c, dJ.d(), J, Q * Q);
+    h2d(c, dP.d(), Pqp, nn * Q);
+    {
+      Timer t(c, "df_fit", 4.0 * (double)Q * Q * Q + 2.0 * (double)Q * Q * nn);
+      Eigen eig(c, naux);
+      d2d(c, U.d(), dJ.d(), Q * Q);
+      eig.run(U.d(), w.d());
+      double dmin = 0.0;
+      FH_HIP(hipMemcpyAsync(&dmin, w.d(), sizeof(double), hipMemcpyDeviceToHost, c.s));
+      FH_HIP(hipStreamSynchronize(c.s));
+      FH_CHECK(dmin > 0.0, -1, "fh_df_fit: Coulomb metric J is not positive definite");
+      scale_cols_rsqrt(c.s, naux, U.d(), w.d(), Us.d());
+      mm(c, false, true, Q, Q, Q, 1.0, Us.d(), Q, U.d(), Q, 0.0, dJ.d(), Q);   // Jh (into dJ)
+      mm(c, true, true, Q, nn, Q, 1.0, dJ.d(), Q, dP.d(), nn, 0.0, db.d(), Q);
+    }
+    d2h(c, b, db.d(), nn * Q);
+    flush_timers(c);
+  });
+}
+
 int fh_get_df(fh_ctx* x, double* Boo, double* Bov, double* Bvv) {
   return guarded([&] {
     FH_CHECK(x != nullptr, -1, "null context");

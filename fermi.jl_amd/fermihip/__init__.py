@@ -34,7 +34,7 @@ __all__ = [
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXPORTED_SYMBOLS = [
     "fh_default_opts", "fh_create", "fh_set_df", "fh_prepare", "fh_t2_guess", "fh_ccsd_energy",
-    "fh_ccsd_residual", "fh_ccsd_solve", "fh_mp2_energy", "fh_set_df_ao", "fh_get_df", "fh_default_scf_opts", "fh_rhf", "fh_triples", "fh_stats", "fh_last_error", "fh_destroy",
+    "fh_ccsd_residual", "fh_ccsd_solve", "fh_mp2_energy", "fh_set_df_ao", "fh_get_df", "fh_default_scf_opts", "fh_rhf", "fh_df_fit", "fh_triples", "fh_stats", "fh_last_error", "fh_destroy",
     "fh_mlp_create", "fh_mlp_num_params", "fh_mlp_get_params", "fh_mlp_set_params", "fh_mlp_forward",
     "fh_mlp_loss_grad", "fh_mlp_train", "fh_mlp_destroy", "fh_set_t2_model",
 ]
@@ -119,6 +119,7 @@ def load_library():
     L.fh_mp2_energy.argtypes = [ctypes.c_void_p, dp]
     L.fh_set_df_ao.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [dp, dp, dp]
     L.fh_get_df.argtypes = [ctypes.c_void_p, dp, dp, dp]
+    L.fh_df_fit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, dp, dp]
     L.fh_default_scf_opts.argtypes = [ctypes.POINTER(_ScfOpts)]
     L.fh_default_scf_opts.restype = None
     L.fh_rhf.argtypes = [ctypes.c_void_p, ctypes.POINTER(_ScfOpts), ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -268,6 +269,16 @@ class Context:
         self._check(self.L.fh_set_df_ao(self.h, n, Q, int(ndocc), int(drop_occ), int(drop_vir),
                                         *[_ptr(x) for x in keep]))
         self.o, self.v, self.naux = ndocc - drop_occ, n - drop_vir - ndocc, Q
+
+    def df_fit(self, J, Pqp):
+        """b[Q,mu,nu] = Pqp[mu,nu,P] J^(-1/2)[P,Q] (fh_df_fit, AtomicIntegrals.jl:31-41)."""
+        n, n2, Q = Pqp.shape
+        if n2 != n or J.shape != (Q, Q):
+            raise FermiException("inconsistent J / Pqp shapes")
+        keep = [_colmajor(J), _colmajor(Pqp)]
+        b = np.zeros((Q, n, n), order="F")
+        self._check(self.L.fh_df_fit(self.h, n, Q, _ptr(keep[0]), _ptr(keep[1]), _ptr(b)))
+        return b
 
     def rhf(self, S, H, b, ndocc, Vnuc=0.0, options=None):
         """DF-RHF SCF (fh_rhf): returns (result dict, eps[nbf], C[nbf,nbf])."""

@@ -86,6 +86,12 @@ int fh_create(fh_ctx** ctx, int device);
 int fh_set_df(fh_ctx* ctx, int o, int v, int naux, const double* eps_o, const double* eps_v, const double* Boo,
               const double* Bov, const double* Bvv);
 
+/* The DF fit of compute_ERI! for DF ERIs (AtomicIntegrals.jl:31-41): Jh = J^(-1/2), the symmetric
+ * inverse square root of the Coulomb metric J[P,Q] = ERI_2e2c(auxbs) (from its eigenpairs; J must be
+ * positive definite), and b[Q,mu,nu] = sum_P Pqp[mu,nu,P] Jh[P,Q] with Pqp = ERI_2e3c(bs, auxbs)
+ * [nbf,nbf,naux].  b[naux,nbf,nbf] is a host output (aoints["ERI"]). */
+int fh_df_fit(fh_ctx* ctx, int nbf, int naux, const double* J, const double* Pqp, double* b);
+
 /* AO -> MO DF transforms on the device, then as fh_set_df: compute_BOO! / compute_BOV! /
  * compute_BVV! (DFERI.jl:15-69) and the RHF orbital energies of compute_F! (Fock.jl:81-98).
  * bAO[Q,mu,nu] = aoints["ERI"], the fitted AO tensor b = (mu nu|P) J^-1/2 (AtomicIntegrals.jl:31-41);

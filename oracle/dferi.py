@@ -1,6 +1,7 @@
-"""CPU oracle (test infrastructure only): FP64 restatement of Fermi.jl's AO -> MO DF transforms
-``compute_BOO!`` / ``compute_BOV!`` / ``compute_BVV!`` (DFERI.jl:15-69) and the RHF orbital
-energies of ``compute_F!`` (Fock.jl:81-98).
+"""CPU oracle (test infrastructure only): FP64 restatement of Fermi.jl's DF fit of
+``compute_ERI!`` (AtomicIntegrals.jl:31-41), the AO -> MO DF transforms ``compute_BOO!`` /
+``compute_BOV!`` / ``compute_BVV!`` (DFERI.jl:15-69) and the RHF orbital energies of
+``compute_F!`` (Fock.jl:81-98).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg may import this
 module, and only as the checker.  The product path (``libfermihip.so``) never calls it.
@@ -8,6 +9,9 @@ module, and only as the checker.  The product path (``libfermihip.so``) never ca
 Arrays are indexed like the Julia arrays: ``bAO[Q,mu,nu]``, ``C[mu,p]``, ``B[Q,p,q]``.
 
 Pins (``tests/test_oracle_dferi.py``):
+  * DF fit: the fitted four-index tensor sum_Q b b equals Pqp J^-1 Pqp computed with a linear
+    solve (no square root); J = 1 gives b = Pqp permuted; the fit with J^2 in place of J applies
+    J^-1 (the square root squared);
   * C = identity: every block is the matching slice of bAO;
   * four-index consistency: sum_Q Bov[Q,i,a] Bov[Q,j,b] equals the AO four-index tensor
     (mu nu|la si) = sum_Q b b transformed one index at a time, on an AO tensor that is not symmetric
@@ -17,7 +21,15 @@ Pins (``tests/test_oracle_dferi.py``):
 """
 import numpy as np
 
-__all__ = ["active_spaces", "df_ao_to_mo", "rhf_eps"]
+__all__ = ["active_spaces", "df_ao_to_mo", "rhf_eps", "df_fit"]
+
+
+def df_fit(J, Pqp):
+    """b[Q,p,q] = Pqp[p,q,P] Jh[P,Q] with Jh = real(J^(-1/2)) (AtomicIntegrals.jl:35-40), the
+    symmetric inverse square root of the positive definite Coulomb metric."""
+    d, U = np.linalg.eigh(0.5 * (J + J.T))
+    Jh = (U / np.sqrt(d)) @ U.T
+    return np.einsum("pqP,PQ->Qpq", Pqp, Jh, optimize=True)
 
 
 def active_spaces(nbf, ndocc, drop_occ=0, drop_vir=0):

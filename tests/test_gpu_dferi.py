@@ -1,5 +1,5 @@
-"""GPU parity of the AO -> MO DF transform (fh_set_df_ao / fh_get_df, through the C ABI) against
-the oracle's restatement of DFERI.jl:15-69 (pinned on the CPU by the identity-coefficient case,
+"""GPU parity of the DF fit (fh_df_fit, AtomicIntegrals.jl:31-41) and the AO -> MO DF transform
+(fh_set_df_ao / fh_get_df, through the C ABI) against the oracle's restatement of DFERI.jl:15-69 (pinned on the CPU by the identity-coefficient case,
 four-index consistency and orthogonal invariance).  FP64 MFMA GEMMs: 1e-12 relative.  Downstream,
 a context made from the AO tensor gives the same CCSD update and DF-MP2 energy as one made from
 the oracle's MO tensors.
@@ -9,7 +9,7 @@ import sys
 import numpy as np
 import pytest
 
-from oracle.dferi import df_ao_to_mo, rhf_eps
+from oracle.dferi import df_ao_to_mo, df_fit, rhf_eps
 from oracle.rmp2 import rmp2_energy_df
 from helpers import relerr
 
@@ -80,3 +80,21 @@ def test_ao_to_mo_errors():
         c.set_df_ao(eps, C, b, 3, 0, 5)
     with pytest.raises(fermihip.FermiException, match="occupied orbital energy"):
         c.set_df_ao(eps[::-1].copy(), C, b, 3, 0, 0)
+
+
+@pytest.mark.parametrize("nbf,naux,seed", [(3, 5, 0), (10, 25, 1), (30, 84, 2)])
+def test_df_fit_matches_oracle(nbf, naux, seed):
+    import fermihip
+    from test_oracle_dferi import random_fit
+    say(f"df_fit nbf={nbf} naux={naux}")
+    J, P = random_fit(nbf, naux, seed)
+    b = fermihip.Context(0).df_fit(J, P)
+    assert relerr(b, df_fit(J, P)) < 1e-11
+
+
+def test_df_fit_errors():
+    import fermihip
+    say("df_fit errors")
+    J = -np.eye(4)
+    with pytest.raises(fermihip.FermiException, match="positive definite"):
+        fermihip.Context(0).df_fit(J, np.ones((2, 2, 4)))

@@ -1,9 +1,10 @@
-"""Pins of the AO -> MO DF transform oracle (oracle/dferi.py, a restatement of DFERI.jl:15-69 and
-Fock.jl:81-98): the identity-coefficient special case, four-index consistency on an AO tensor
-that is not symmetric in (mu, nu), and orthogonal invariance of the Frobenius norm."""
+"""Pins of the DF oracle (oracle/dferi.py, a restatement of AtomicIntegrals.jl:31-41, DFERI.jl:15-69
+and Fock.jl:81-98): the identity-coefficient special case, four-index consistency on an AO tensor
+that is not symmetric in (mu, nu), orthogonal invariance of the Frobenius norm, and for the DF fit
+the four-index tensor against a linear solve, the unit metric and the squared metric."""
 import numpy as np
 
-from oracle.dferi import df_ao_to_mo, rhf_eps
+from oracle.dferi import df_ao_to_mo, df_fit, rhf_eps
 
 
 def random_ao(naux, nbf, seed, symmetric=True):
@@ -50,3 +51,29 @@ def test_orthogonal_invariance():
     Boo, Bov, Bvv = df_ao_to_mo(b, C, 3)
     lhs = np.sum(Boo ** 2) + 2.0 * np.sum(Bov ** 2) + np.sum(Bvv ** 2)
     assert abs(lhs - np.sum(b ** 2)) < 1e-11 * np.sum(b ** 2)
+
+
+def random_fit(nbf, naux, seed):
+    """Positive definite Coulomb metric J and a 3-center tensor Pqp[mu,nu,P] symmetric in (mu, nu)."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((naux, naux))
+    J = A @ A.T / naux + 0.5 * np.eye(naux)
+    P = rng.standard_normal((nbf, nbf, naux))
+    return J, P + np.transpose(P, (1, 0, 2))
+
+
+def test_df_fit_pins():
+    J, P = random_fit(6, 11, 5)
+    b = df_fit(J, P)
+    # sum_Q b b = Pqp J^-1 Pqp, evaluated with a linear solve instead of the square root
+    four = np.einsum("Qmn,Qls->mnls", b, b)
+    X = np.linalg.solve(J, P.reshape(36, 11).T)                     # J^-1 Pqp^T  [P, (l,s)]
+    ref = (P.reshape(36, 11) @ X).reshape(6, 6, 6, 6)
+    assert np.max(np.abs(four - ref)) < 1e-11 * np.max(np.abs(ref))
+    # J = 1: b[Q,m,n] = Pqp[m,n,Q]
+    assert np.max(np.abs(df_fit(np.eye(11), P) - np.transpose(P, (2, 0, 1)))) < 1e-14
+    # the square root itself: b for J and J^2 relate through Jh(J^2) = Jh(J)^2
+    b2 = df_fit(J @ J, P)
+    d, U = np.linalg.eigh(J)
+    Jm = (U / d) @ U.T                                               # J^-1
+    assert np.max(np.abs(b2 - np.einsum("pqP,PQ->Qpq", P, Jm))) < 1e-11 * np.max(np.abs(b2))
