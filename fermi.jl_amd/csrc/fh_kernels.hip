@@ -65,6 +65,33 @@ __global__ __launch_bounds__(256) void k_permute_simple(PermArgs a, int64_t n, d
   }
 }
 
+// The same with 32-bit index arithmetic (tensors of fewer than 2^31 elements, input offsets below
+// 2^31): each coordinate comes from a division by an invariant integer done as a multiply-high and a
+// shift (Granlund & Montgomery, PLDI 1994: q = (umulhi(n, m) + n) >> s with s = ceil(log2 d) and
+// m = floor(2^32 (2^s - d) / d) + 1, exact for n < 2^31), instead of six 64-bit divisions per element.
+struct PermArgs32 {
+  int nd;
+  uint32_t odim[6], istr[6], m[6], sh[6];
+};
+__global__ __launch_bounds__(256) void k_permute_simple32(PermArgs32 a, uint32_t n, double alpha,
+                                                          const double* __restrict__ in, double beta,
+                                                          double* __restrict__ out) {
+  for (uint32_t idx = blockIdx.x * 256u + threadIdx.x; idx < n; idx += gridDim.x * 256u) {
+    uint32_t rem = idx, off = 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      if (d < a.nd) {
+        const uint32_t q = (__umulhi(rem, a.m[d]) + rem) >> a.sh[d];
+        off += (rem - q * a.odim[d]) * a.istr[d];
+        rem = q;
+      }
+    }
+    double v = alpha * in[off];
+    if (beta != 0.0) v += beta * out[idx];
+    out[idx] = v;
+  }
+}
+
 // 32x32 tiles over (output dim 0, output dim p) where dim p is the input's unit-stride dim:
 // reads are coalesced along p, writes along output dim 0.
 __global__ __launch_bounds__(256) void k_permute_tiled(PermArgs a, int p, int64_t nrest, int64_t t0, int64_t tp,
@@ -126,7 +153,26 @@ void permute_f64(hipStream_t s, int nd, const int64_t* odim, const int64_t* istr
   if (p < 0 || a.odim[0] < 8) {
     int64_t g = (n + 255) / 256;
     if (g > 65536 * 4) g = 65536 * 4;
-    hipLaunchKernelGGL(k_permute_simple, dim3((unsigned)g), dim3(256), 0, s, a, n, alpha, in, beta, out);
+    int64_t maxoff = 0;
+    for (int d = 0; d < nd; ++d) maxoff += (a.odim[d] - 1) * a.istr[d];
+    bool small = n < (int64_t(1) << 31) && maxoff < (int64_t(1) << 31);
+    for (int d = 0; d < nd; ++d) small = small && a.istr[d] >= 0;
+    if (small) {
+      PermArgs32 b;
+      b.nd = nd;
+      for (int d = 0; d < 6; ++d) {
+        const uint32_t dv = d < nd ? (uint32_t)a.odim[d] : 1u;
+        uint32_t sh = 0;
+        while ((uint64_t(1) << sh) < dv) ++sh;
+        b.odim[d] = dv;
+        b.istr[d] = d < nd ? (uint32_t)a.istr[d] : 0u;
+        b.sh[d] = sh;
+        b.m[d] = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << sh) - dv)) / dv + 1);
+      }
+      hipLaunchKernelGGL(k_permute_simple32, dim3((unsigned)g), dim3(256), 0, s, b, (uint32_t)n, alpha, in, beta, out);
+    } else {
+      hipLaunchKernelGGL(k_permute_simple, dim3((unsigned)g), dim3(256), 0, s, a, n, alpha, in, beta, out);
+    }
   } else {
     const int64_t t0 = (a.odim[0] + 31) / 32, tp = (a.odim[p] + 31) / 32;
     const int64_t nrest = n / (a.odim[0] * a.odim[p]);
