@@ -358,8 +358,49 @@ __global__ __launch_bounds__(RB) void k_cc_energy(int o, int v, const double* __
   const double t = block_sum(acc);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
+// The same sum for o <= 64, one (c,d) pair at a time: the o x o tile T2[:,:,c,d] is read once,
+// contiguously, into LDS (rows padded to o+1), where both T2[k,l,c,d] and T2[l,k,c,d] are read;
+// OVOV[k,c,l,d] is read in runs of o along k.  A fixed grid strides over the pairs.
+static constexpr int kCcTileO = 64;
+__global__ __launch_bounds__(RB) void k_cc_energy_tiles(int o, int v, const double* __restrict__ t1,
+                                                        const double* __restrict__ t2,
+                                                        const double* __restrict__ ovov,
+                                                        double* __restrict__ partials) {
+  __shared__ double tile[kCcTileO * (kCcTileO + 1)];
+  __shared__ double tc[kCcTileO], td[kCcTileO];
+  const int oo = o * o, op = o + 1;
+  const int64_t ov = (int64_t)o * v;
+  double acc = 0.0;
+  for (int cd = blockIdx.x; cd < v * v; cd += gridDim.x) {
+    const int c = cd % v, d = cd / v;
+    const double* __restrict__ t2cd = t2 + (int64_t)oo * cd;            // T2[:, :, c, d]
+    __syncthreads();                                                     // previous pair's reads done
+    for (int e = threadIdx.x; e < oo; e += RB) tile[(e % o) + op * (e / o)] = t2cd[e];
+    if ((int)threadIdx.x < o) {
+      tc[threadIdx.x] = t1[threadIdx.x + o * c];
+      td[threadIdx.x] = t1[threadIdx.x + o * d];
+    }
+    __syncthreads();
+    const double* __restrict__ vcd = ovov + (int64_t)o * c + ov * o * d;  // + k + o v l
+    for (int e = threadIdx.x; e < oo; e += RB) {
+      const int k = e % o, l = e / o;
+      const double b = 2.0 * tile[k + op * l] - tile[l + op * k] - tc[l] * td[k] + 2.0 * tc[k] * td[l];
+      acc += b * vcd[k + ov * l];
+    }
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
 void cc_energy(hipStream_t s, int o, int v, const double* t1, const double* t2, const double* ovov, double* partials,
                double* result) {
+  if (o <= kCcTileO) {
+    const int64_t g = std::min<int64_t>((int64_t)v * v, RMAX);
+    hipLaunchKernelGGL(k_cc_energy_tiles, dim3(g), dim3(RB), 0, s, o, v, t1, t2, ovov, partials);
+    FH_HIP(hipGetLastError());
+    sum_partials(s, g, partials, result);
+    return;
+  }
   const int64_t n = (int64_t)o * v * o * v;
   const int64_t g = reduce_blocks_needed(n);
   hipLaunchKernelGGL(k_cc_energy, dim3(g), dim3(RB), 0, s, o, v, t1, t2, ovov, partials);
