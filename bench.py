@@ -134,7 +134,8 @@ def main():
                          "checked against the plain FP64 run of the same workload (error_vs_fp64)")
     ap.add_argument("--polish-iters", type=int, default=2,
                     help="FP64 polish iterations after the reduced-precision phases (-1: to convergence)")
-    ap.add_argument("--bf3-switch", default="1e-7,1e-8", help="|dE|,rms that end the split-BF16x3 phase")
+    ap.add_argument("--bf3-switch", default="1e-6,1e-7",
+                    help="|dE|,rms that end the split-BF16x3 phase (the library default is 1e-7,1e-8)")
     ap.add_argument("--mixed-switch", default="1e-9,1e-9", help="|dE|,rms that end the FP32 phase (--precision mixed)")
     ap.add_argument("--verbose", action="store_true", help="print the CC iteration table")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "off"])
