@@ -124,7 +124,7 @@ def main():
     ap.add_argument("--guess", default="auto", choices=["auto", "learned", "mp2"],
                     help="initial T2: learned (the committed network; auto = learned when the model file exists) "
                          "or mp2")
-    ap.add_argument("--t1-guess", default="zero", choices=["zero", "first-order"],
+    ap.add_argument("--t1-guess", default="first-order", choices=["zero", "first-order"],
                     help="initial T1: zero (the reference's) or the first-order T1 at the guess's T2")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--diis-prec", default="single", choices=["single", "double"])
