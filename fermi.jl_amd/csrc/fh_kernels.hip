@@ -93,38 +93,56 @@ __global__ __launch_bounds__(256) void k_permute_simple32(PermArgs32 a, uint32_t
 }
 
 // 32x32 tiles over (output dim 0, output dim p) where dim p is the input's unit-stride dim:
-// reads are coalesced along p, writes along output dim 0.
+// reads are coalesced along p, writes along output dim 0.  Each workgroup iteration moves two tiles
+// (eight loads per thread in flight before the barrier).
 __global__ __launch_bounds__(256) void k_permute_tiled(PermArgs a, int p, int64_t nrest, int64_t t0, int64_t tp,
                                                        double alpha, const double* __restrict__ in, double beta,
                                                        double* __restrict__ out) {
-  __shared__ double tile[32][33];
+  __shared__ double tile[2][32][33];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int64_t blk = blockIdx.x; blk < nrest * t0 * tp; blk += gridDim.x) {
-    const int64_t b0 = blk % t0, bp = (blk / t0) % tp, rest = blk / (t0 * tp);
-    // decompose the remaining dims
-    int64_t rem = rest, ioff = 0, ooff = 0;
-    for (int d = 1; d < a.nd; ++d) {
-      if (d == p) continue;
-      const int64_t c = rem % a.odim[d];
-      rem /= a.odim[d];
-      ioff += c * a.istr[d];
-      ooff += c * a.ostr[d];
+  const int64_t ntiles = nrest * t0 * tp;
+  for (int64_t base = 2 * (int64_t)blockIdx.x; base < ntiles; base += 2 * (int64_t)gridDim.x) {
+    int64_t b0[2], bp[2], ioff[2], ooff[2];
+    bool live[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      live[u] = base + u < ntiles;
+      const int64_t blk = live[u] ? base + u : base;
+      b0[u] = blk % t0;
+      bp[u] = (blk / t0) % tp;
+      // decompose the remaining dims
+      int64_t rem = blk / (t0 * tp), io = 0, oo = 0;
+      for (int d = 1; d < a.nd; ++d) {
+        if (d == p) continue;
+        const int64_t c = rem % a.odim[d];
+        rem /= a.odim[d];
+        io += c * a.istr[d];
+        oo += c * a.ostr[d];
+      }
+      ioff[u] = io;
+      ooff[u] = oo;
     }
     // load: threads consecutive along p
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t i0 = b0 * 32 + ty + 8 * r, ip = bp * 32 + tx;
-      if (i0 < a.odim[0] && ip < a.odim[p]) tile[ty + 8 * r][tx] = in[ioff + i0 * a.istr[0] + ip];
+    for (int u = 0; u < 2; ++u) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i0 = b0[u] * 32 + ty + 8 * r, ip = bp[u] * 32 + tx;
+        if (live[u] && i0 < a.odim[0] && ip < a.odim[p]) tile[u][ty + 8 * r][tx] = in[ioff[u] + i0 * a.istr[0] + ip];
+      }
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t i0 = b0 * 32 + tx, ip = bp * 32 + ty + 8 * r;
-      if (i0 < a.odim[0] && ip < a.odim[p]) {
-        const int64_t o = ooff + i0 + ip * a.ostr[p];
-        double v = alpha * tile[tx][ty + 8 * r];
-        if (beta != 0.0) v += beta * out[o];
-        out[o] = v;
+    for (int u = 0; u < 2; ++u) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i0 = b0[u] * 32 + tx, ip = bp[u] * 32 + ty + 8 * r;
+        if (live[u] && i0 < a.odim[0] && ip < a.odim[p]) {
+          const int64_t o = ooff[u] + i0 + ip * a.ostr[p];
+          double v = alpha * tile[u][tx][ty + 8 * r];
+          if (beta != 0.0) v += beta * out[o];
+          out[o] = v;
+        }
       }
     }
     __syncthreads();
@@ -176,7 +194,7 @@ void permute_f64(hipStream_t s, int nd, const int64_t* odim, const int64_t* istr
   } else {
     const int64_t t0 = (a.odim[0] + 31) / 32, tp = (a.odim[p] + 31) / 32;
     const int64_t nrest = n / (a.odim[0] * a.odim[p]);
-    int64_t g = nrest * t0 * tp;
+    int64_t g = (nrest * t0 * tp + 1) / 2;
     if (g > 65536 * 4) g = 65536 * 4;
     hipLaunchKernelGGL(k_permute_tiled, dim3((unsigned)g), dim3(256), 0, s, a, p, nrest, t0, tp, alpha, in, beta, out);
   }
