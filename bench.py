@@ -149,7 +149,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this DF-RCCSD(T) engine runs on one GPU; multi-GPU runs are not part of this build "
-              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+              "(the pair-sharded decomposition is a design, not code).", file=sys.stderr)
         sys.exit(2)
 
     if args.phase == "copy":
