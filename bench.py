@@ -263,6 +263,20 @@ def main():
             df_rhf["df_fit_ms"] = p_fit["ms"]
             log(f"bench: DF fit naux={Q} nbf={nbf} {p_fit['ms']:.2f} ms on the device")
         del P_f
+        # conventional 4-index transform (fh_ao2mo_4index, FP64) of a random AO tensor with the
+        # 96 basis functions of (H2O)4 in cc-pVDZ, all four indices over all MOs (compute_ERI!)
+        nc = 96
+        eri_c = rng.standard_normal((nc, nc, nc, nc))
+        C_c = rng.standard_normal((nc, nc))
+        ctx.ao2mo_4index(eri_c, C_c, C_c, C_c, C_c)
+        b_ph = ctx.stats()["phases"]
+        ctx.ao2mo_4index(eri_c, C_c, C_c, C_c, C_c)
+        p_c = phase_diff(ctx.stats()["phases"], b_ph).get("ao2mo_4index")
+        if df_ao is not None and p_c and p_c["ms"] > 0:
+            df_ao["conventional_4index"] = {"nbf": nc, "ms": p_c["ms"],
+                                            "tflops": p_c["flops"] / (p_c["ms"] * 1e-3) / 1e12}
+            log(f"bench: conventional 4-index transform nbf={nc} {p_c['ms']:.2f} ms on the device")
+        del eri_c
         return df_mp2, df_ao, df_rhf
 
     def step():

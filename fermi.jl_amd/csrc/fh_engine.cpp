@@ -1495,6 +1495,47 @@ int fh_df_fit(fh_ctx* x, int nbf, int naux, const double* J, const double* Pqp, 
   });
 }
 
+// Conventional 4-index AO -> MO transform (Chonky.jl:1-131): out[i,j,k,l] = eri[mu,nu,rho,sigma]
+// C1[mu,i] C2[nu,j] C3[rho,k] C4[sigma,l] as four quarter transforms, each one FP64 MFMA GEMM (the
+// middle two strided-batched over the indices not yet transformed).
+int fh_ao2mo_4index(fh_ctx* x, int nbf, int n1, const double* C1, int n2, const double* C2, int n3, const double* C3,
+                    int n4, const double* C4, const double* eri, double* out) {
+  return guarded([&] {
+    FH_CHECK(x != nullptr && C1 && C2 && C3 && C4 && eri && out, -1, "fh_ao2mo_4index: null argument");
+    FH_CHECK(nbf >= 1 && n1 >= 1 && n2 >= 1 && n3 >= 1 && n4 >= 1 && n1 <= nbf && n2 <= nbf && n3 <= nbf && n4 <= nbf,
+             -1, "fh_ao2mo_4index: invalid dimensions");
+    const int64_t n = nbf, nn = n * n, n3a = nn * n, n4a = nn * nn;
+    FH_CHECK((double)n4a < 9.0e18 / 8, -1, "fh_ao2mo_4index: AO tensor too large");
+    DeviceGuard g(x->c.device);
+    Ctx& c = x->c;
+    auto up = [&](const double* h, int64_t cnt) { Buf b = dalloc(c, cnt); h2d(c, b.d(), h, cnt); return b; };
+    Buf dC1 = up(C1, n * n1), dC2 = up(C2, n * n2), dC3 = up(C3, n * n3), dC4 = up(C4, n * n4), dE = up(eri, n4a);
+    const int64_t m12 = (int64_t)n1 * n2, m123 = m12 * n3, nout = m123 * n4;
+    Buf X1 = dalloc(c, (int64_t)n1 * n3a), X2 = dalloc(c, m12 * nn), X3 = dalloc(c, m123 * n), dO = dalloc(c, nout);
+    {
+      Timer t(c, "ao2mo_4index", 2.0 * ((double)n1 * n4a + (double)m12 * n3a + (double)m123 * nn + (double)nout * n));
+      auto gemm = [&](bool ta, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, int64_t sA, const double* B,
+                      int64_t ldb, double* D, int64_t ldd, int64_t sD, int64_t batch) {
+        const int64_t wsz = gemm_splitk_elems(M, N, K, batch);
+        Buf work;
+        if (wsz) work = dalloc(c, wsz);
+        gemm_f64(c.s, ta, false, M, N, K, 1.0, A, lda, sA, B, ldb, 0, 0.0, nullptr, 0, 0, D, ldd, sD, batch, 0,
+                 work.p ? work.d() : nullptr, wsz);
+      };
+      // X1[i,(nu rho sigma)] = C1[mu,i] eri[mu,(nu rho sigma)]
+      gemm(true, n1, n3a, n, dC1.d(), n, 0, dE.d(), n, X1.d(), n1, 0, 1);
+      // X2[(i,j),(rho sigma)] = X1[i,nu,(rho sigma)] C2[nu,j], one batch entry per (rho, sigma)
+      gemm(false, n1, n2, n, X1.d(), n1, (int64_t)n1 * n, dC2.d(), n, X2.d(), n1, m12, nn);
+      // X3[(i,j,k),sigma] = X2[(i,j),rho,sigma] C3[rho,k], one batch entry per sigma
+      gemm(false, m12, n3, n, X2.d(), m12, m12 * n, dC3.d(), n, X3.d(), m12, m123, n);
+      // out[(i,j,k),l] = X3[(i,j,k),sigma] C4[sigma,l]
+      gemm(false, m123, n4, n, X3.d(), m123, 0, dC4.d(), n, dO.d(), m123, 0, 1);
+    }
+    d2h(c, out, dO.d(), nout);
+    flush_timers(c);
+  });
+}
+
 int fh_get_df(fh_ctx* x, double* Boo, double* Bov, double* Bvv) {
   return guarded([&] {
     FH_CHECK(x != nullptr, -1, "null context");

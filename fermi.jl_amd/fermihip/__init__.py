@@ -34,7 +34,7 @@ __all__ = [
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXPORTED_SYMBOLS = [
     "fh_default_opts", "fh_create", "fh_set_df", "fh_prepare", "fh_t2_guess", "fh_ccsd_energy",
-    "fh_ccsd_residual", "fh_ccsd_solve", "fh_mp2_energy", "fh_set_df_ao", "fh_get_df", "fh_default_scf_opts", "fh_rhf", "fh_df_fit", "fh_triples", "fh_stats", "fh_last_error", "fh_destroy",
+    "fh_ccsd_residual", "fh_ccsd_solve", "fh_mp2_energy", "fh_set_df_ao", "fh_get_df", "fh_default_scf_opts", "fh_rhf", "fh_df_fit", "fh_ao2mo_4index", "fh_triples", "fh_stats", "fh_last_error", "fh_destroy",
     "fh_mlp_create", "fh_mlp_num_params", "fh_mlp_get_params", "fh_mlp_set_params", "fh_mlp_forward",
     "fh_mlp_loss_grad", "fh_mlp_train", "fh_mlp_destroy", "fh_set_t2_model",
 ]
@@ -120,6 +120,8 @@ def load_library():
     L.fh_set_df_ao.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [dp, dp, dp]
     L.fh_get_df.argtypes = [ctypes.c_void_p, dp, dp, dp]
     L.fh_df_fit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, dp, dp]
+    L.fh_ao2mo_4index.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, dp,
+                                  ctypes.c_int, dp, dp, dp]
     L.fh_default_scf_opts.argtypes = [ctypes.POINTER(_ScfOpts)]
     L.fh_default_scf_opts.restype = None
     L.fh_rhf.argtypes = [ctypes.c_void_p, ctypes.POINTER(_ScfOpts), ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -279,6 +281,19 @@ class Context:
         b = np.zeros((Q, n, n), order="F")
         self._check(self.L.fh_df_fit(self.h, n, Q, _ptr(keep[0]), _ptr(keep[1]), _ptr(b)))
         return b
+
+    def ao2mo_4index(self, eri, C1, C2, C3, C4):
+        """Conventional 4-index transform (fh_ao2mo_4index, Chonky.jl:1-131):
+        out[i,j,k,l] = eri[mu,nu,rho,sigma] C1[mu,i] C2[nu,j] C3[rho,k] C4[sigma,l]."""
+        n = eri.shape[0]
+        if eri.shape != (n, n, n, n) or any(c.shape[0] != n for c in (C1, C2, C3, C4)):
+            raise FermiException("inconsistent ERI / coefficient shapes")
+        keep = [_colmajor(x) for x in (C1, C2, C3, C4, eri)]
+        out = np.zeros((C1.shape[1], C2.shape[1], C3.shape[1], C4.shape[1]), order="F")
+        self._check(self.L.fh_ao2mo_4index(self.h, n, C1.shape[1], _ptr(keep[0]), C2.shape[1], _ptr(keep[1]),
+                                           C3.shape[1], _ptr(keep[2]), C4.shape[1], _ptr(keep[3]), _ptr(keep[4]),
+                                           _ptr(out)))
+        return out
 
     def rhf(self, S, H, b, ndocc, Vnuc=0.0, options=None):
         """DF-RHF SCF (fh_rhf): returns (result dict, eps[nbf], C[nbf,nbf])."""

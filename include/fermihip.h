@@ -135,6 +135,15 @@ void fh_default_scf_opts(fh_scf_opts* opts);
 int fh_rhf(fh_ctx* ctx, const fh_scf_opts* opts, int nbf, int naux, int ndocc, double Vnuc, const double* S,
            const double* H, const double* bAO, double* eps, double* C, fh_scf_result* res);
 
+/* Conventional (non-DF) 4-index AO -> MO transform of the Chonky integral path (Chonky.jl:1-131):
+ * out[i,j,k,l] = sum eri[mu,nu,rho,sigma] C1[mu,i] C2[nu,j] C3[rho,k] C4[sigma,l], eri[nbf^4] =
+ * aoints["ERI"] (chemists' order), Cn[nbf,nn] column blocks of orbitals.C.  compute_ERI! is
+ * C1..C4 = C; compute_OOOO! ... compute_VVVV! use the occupied (columns drop_occ+1 : Nalpha) and
+ * virtual (Nalpha+1 : nbf-drop_vir) blocks in the order of their names.  out[n1,n2,n3,n4] is a host
+ * output.  FP64 MFMA GEMMs. */
+int fh_ao2mo_4index(fh_ctx* ctx, int nbf, int n1, const double* C1, int n2, const double* C2, int n3, const double* C3,
+                    int n4, const double* C4, const double* eri, double* out);
+
 /* Copy the context's MO DF tensors back to the host (the moints cache entries "BOO", "BOV", "BVV"
  * that DFERI.jl:29,50,68 store); any pointer may be NULL. */
 int fh_get_df(fh_ctx* ctx, double* Boo, double* Bov, double* Bvv);

@@ -98,3 +98,21 @@ def test_df_fit_errors():
     J = -np.eye(4)
     with pytest.raises(fermihip.FermiException, match="positive definite"):
         fermihip.Context(0).df_fit(J, np.ones((2, 2, 4)))
+
+
+@pytest.mark.parametrize("nbf,nd,core,inac,seed", [(5, 2, 0, 0, 0), (14, 5, 1, 2, 1), (24, 5, 0, 0, 2)])
+def test_ao2mo_4index_matches_oracle(nbf, nd, core, inac, seed):
+    """Conventional transform (Chonky.jl:1-131): the full MO tensor and every block."""
+    import fermihip
+    from oracle.chonky import ao2mo_4index, chonky_block
+    say(f"ao2mo_4index nbf={nbf}")
+    rng = np.random.default_rng(seed)
+    eri = rng.standard_normal((nbf, nbf, nbf, nbf))
+    C = rng.standard_normal((nbf, nbf))
+    c = fermihip.Context(0)
+    assert relerr(c.ao2mo_4index(eri, C, C, C, C), ao2mo_4index(eri, C, C, C, C)) < 1e-12
+    o, v = slice(core, nd), slice(nd, nbf - inac)
+    blk = {"O": C[:, o], "V": C[:, v]}
+    for key in ("OOOO", "OOOV", "OOVV", "OVOV", "OVVV", "VVVV"):
+        got = c.ao2mo_4index(eri, *[blk[ch] for ch in key])
+        assert relerr(got, chonky_block(eri, C, nd, key, core, inac)) < 1e-12, key
